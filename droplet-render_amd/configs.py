@@ -1,0 +1,98 @@
+"""Synthetic workloads of the benchmark configurations.
+
+Each builder returns a `Workload`: the density grid, the octree scene, camera,
+samplers and render settings. Densities are procedural (no datasets exist
+offline); the lighting follows the reference's defaults (sun colour (1,1,1),
+intensity 1, angle 0.010 rad, addon/panel.py:360-362) with the sun at 30
+degrees elevation, Mie phase, and a Hosek sky of turbidity 3 and ground albedo 0.1.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .camera import camera, sun_direction
+from .sampler import HosekSky, MiePhase, SunLight
+from .scene import Scene
+
+EYE = (0.5, -1.2, 0.6)
+TARGET = (0.5, 0.5, 0.5)
+FOV_Y = 40.0
+
+
+def synth_fbm(n, seed, octaves=4, freq=4.0, lacunarity=2.0, gain=0.5, threshold=0.3):
+    """fBm gradient-noise cloud: max(f - threshold, 0) normalised to max 1."""
+    out = np.empty((n, n, n), dtype=np.float32)
+    prm = np.array([octaves, freq, lacunarity, gain, threshold], dtype=np.float32)
+    L.check(L.lib().drk_synth_cloud(0, n, int(seed), L.fptr(prm), len(prm), L.fptr(out)))
+    return out
+
+
+def synth_spheres(n, seed, spheres=64, cmin=0.2, cmax=0.8, rmin=0.05, rmax=0.2, amp=0.15, octaves=5, freq=4.0,
+                  band=None):
+    """Union of seeded spheres displaced by fBm; density ramps over `band` (default 2 voxels)."""
+    out = np.empty((n, n, n), dtype=np.float32)
+    band = 2.0 / n if band is None else band
+    prm = np.array([spheres, cmin, cmax, rmin, rmax, amp, octaves, freq, band], dtype=np.float32)
+    L.check(L.lib().drk_synth_cloud(1, n, int(seed), L.fptr(prm), len(prm), L.fptr(out)))
+    return out
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: np.ndarray
+    scene: Scene
+    w: int
+    h: int
+    spp: int
+    scattevs: int
+    sigma_t: float
+    albedo: float = 0.99
+    phase: object = field(default_factory=MiePhase)
+    env: object = field(default_factory=lambda: HosekSky(3.0, 0.1))
+    lights: list = field(default_factory=lambda: [SunLight(sun_direction(30.0, 0.0))])
+    viewi: np.ndarray = None
+    proji: np.ndarray = None
+
+    def __post_init__(self):
+        if self.viewi is None:
+            self.viewi, self.proji = camera(EYE, TARGET, FOV_Y, self.w, self.h)
+
+    @property
+    def msigmas(self):
+        return self.sigma_t * self.albedo
+
+    @property
+    def msigmaa(self):
+        return self.sigma_t * (1.0 - self.albedo)
+
+    @property
+    def samples(self):
+        return self.w * self.h * self.spp
+
+    def kernel(self, tilex=None, tiley=None):
+        from .kernel import RenderKernel
+        k = RenderKernel()
+        k.Initialize(self.scene, self.viewi, self.proji, self.phase, self.env, None, self.scattevs, self.msigmas,
+                     self.msigmaa, tilex or self.w, tiley or self.h, self.w, self.h, 0, self.lights)
+        return k
+
+
+def c0(w=256, h=256, spp=16):
+    """64^3 fBm cloud (4 octaves, f 4, lacunarity 2, gain 0.5, seed 1, threshold 0.3), sigma_t 8, 8 bounces."""
+    g = synth_fbm(64, 1)
+    return Workload("c0_fbm64", g, Scene.from_grid(g), w, h, spp, 8, 8.0)
+
+
+def c1(w=1024, h=576, spp=64):
+    """256^3 sphere-union cloud (64 spheres, 5-octave fBm displacement 0.15), sigma_t 16, 32 bounces."""
+    g = synth_spheres(256, 1)
+    return Workload("c1_spheres256", g, Scene.from_grid(g), w, h, spp, 32, 16.0)
+
+
+def tiny(n=32, w=32, h=32, spp=8, scattevs=4, sigma_t=8.0, seed=3):
+    """Small fBm scene for parity tests that the CPU oracle finishes in seconds."""
+    g = synth_fbm(n, seed, octaves=3, freq=3.0, threshold=0.2)
+    return Workload("tiny_fbm%d" % n, g, Scene.from_grid(g), w, h, spp, scattevs, sigma_t)

@@ -1,0 +1,469 @@
+// extern "C" boundary: argument checking, device memory, streams and the
+// RenderKernel life cycle (src/kernel.cpp:497-615) around the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "drk_common.h"
+#include "drk_internal.h"
+#include "render_launch.h"
+
+namespace drk {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+    return set_error(e == hipErrorOutOfMemory ? DRK_ENOMEM : DRK_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                      \
+    do {                                         \
+        hipError_t _e = (expr);                  \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+}  // namespace drk
+
+using namespace drk;
+
+struct drk_scene {
+    std::vector<drk_octree_node> h_ob;
+    drk_octree_node *d_ob = nullptr;
+    float *d_fog = nullptr;
+    float *d_sdf = nullptr;
+    uint32_t node_count = 0;
+    uint32_t leafx[2] = {0, 0};
+    uint32_t lvoxc = 0;
+    int device = 0;
+};
+
+struct drk_kernel {
+    const drk_scene *scene = nullptr;
+    RenderParams P;                 // launch template (tile fields filled per call)
+    hipStream_t stream = nullptr;
+    uint32_t tilex = 0, tiley = 0;
+    uint64_t seed = 0x5eedULL;
+    uint32_t sample_base = 0;
+    float *d_out = nullptr;         // 4 buffers of tilex*tiley float4
+    float *h_phb = nullptr;         // host copies, same layout
+    float *d_partial = nullptr;
+    size_t partial_bytes = 0;
+    uint32_t *d_counter = nullptr;
+    unsigned long long *d_counts = nullptr;
+    float *d_env = nullptr;
+    float *d_depth = nullptr;
+    uint32_t cu_count = 0;
+    uint32_t blocks_per_cu = 1;
+    uint32_t chunk_override = 0;
+    uint32_t grid_override = 0;
+    unsigned long long last_counts[2] = {0, 0};
+};
+
+extern "C" {
+
+const char *drk_last_error(void) { return g_err.c_str(); }
+
+int drk_device_info(char *name, uint32_t name_len, uint32_t *cu_count) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = (uint32_t)prop.multiProcessorCount;
+    return DRK_OK;
+}
+
+int drk_scene_build_dense(const float *fog, const float *sdf, uint32_t nx, uint32_t ny, uint32_t nz, float voxel_size,
+                          uint32_t maxd, drk_host_scene **out) {
+    return scene_build_dense(fog, sdf, nx, ny, nz, voxel_size, maxd, out);
+}
+
+void drk_host_scene_free(drk_host_scene *scene) { host_scene_free(scene); }
+
+int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float *pvol_sdf, uint32_t leafx_sdf,
+                     const float *pvol_fog, uint32_t leafx_fog, uint32_t lvoxc, drk_scene **out) {
+    if (!out) return set_error(DRK_EINVAL, "drk_scene_create: out is NULL");
+    *out = nullptr;
+    if (!ob || node_count == 0 || node_count > kMaxNodes || lvoxc < 2 || lvoxc > 256)
+        return set_error(DRK_EINVAL, "drk_scene_create: invalid octree (nodes=%u lvoxc=%u)", node_count, lvoxc);
+    if (leafx_fog && !pvol_fog) return set_error(DRK_EINVAL, "drk_scene_create: FOG bricks missing");
+    if (leafx_sdf || pvol_sdf) return set_error(DRK_EINVAL, "drk_scene_create: SDF bricks are not supported yet");
+    // Validate the structure the kernels index blindly.
+    for (uint32_t i = 0; i < node_count; ++i) {
+        const drk_octree_node &n = ob[i];
+        for (int c = 0; c < 8; ++c)
+            if (n.chn[c] >= node_count || (n.chn[c] != 0 && n.chn[c] <= i))
+                return set_error(DRK_EINVAL, "drk_scene_create: node %u child %d out of range", i, c);
+        if (n.volx[DRK_VOLUME_SDF] != DRK_NO_BRICK)
+            return set_error(DRK_EINVAL, "drk_scene_create: node %u has an SDF brick (unsupported)", i);
+        if (n.volx[DRK_VOLUME_FOG] != DRK_NO_BRICK && n.volx[DRK_VOLUME_FOG] >= leafx_fog)
+            return set_error(DRK_EINVAL, "drk_scene_create: node %u brick %u >= leafx %u", i, n.volx[DRK_VOLUME_FOG], leafx_fog);
+        if (!(n.ce[3] > 0.0f)) return set_error(DRK_EINVAL, "drk_scene_create: node %u has non-positive extent", i);
+    }
+    // Interior nodes must sit above level kMaxDepth: the traversal keeps one
+    // stack word per interior level. Children have larger indices than their
+    // parents, so one forward pass assigns every level.
+    {
+        std::vector<uint8_t> lvl(node_count, 0xff);
+        lvl[0] = 0;
+        for (uint32_t i = 0; i < node_count; ++i) {
+            if (lvl[i] == 0xff) continue;
+            const drk_octree_node &n = ob[i];
+            bool leaf = n.volx[0] != DRK_NO_BRICK || n.volx[1] != DRK_NO_BRICK;
+            if (leaf) continue;
+            if (lvl[i] >= kMaxDepth) return set_error(DRK_EINVAL, "drk_scene_create: octree deeper than %u levels", kMaxDepth);
+            for (int c = 0; c < 8; ++c)
+                if (n.chn[c]) lvl[n.chn[c]] = (uint8_t)(lvl[i] + 1);
+        }
+    }
+    drk_scene *s = new (std::nothrow) drk_scene;
+    if (!s) return set_error(DRK_ENOMEM, "drk_scene_create: host allocation");
+    s->node_count = node_count;
+    s->h_ob.assign(ob, ob + node_count);
+    s->leafx[0] = 0;
+    s->leafx[1] = leafx_fog;
+    s->lvoxc = lvoxc;
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipMalloc(&s->d_ob, sizeof(drk_octree_node) * node_count);
+    if (e == hipSuccess) e = hipMemcpy(s->d_ob, ob, sizeof(drk_octree_node) * node_count, hipMemcpyHostToDevice);
+    size_t nb = (size_t)leafx_fog * lvoxc * lvoxc * lvoxc;
+    if (e == hipSuccess && nb) e = hipMalloc(&s->d_fog, nb * sizeof(float));
+    if (e == hipSuccess && nb) e = hipMemcpy(s->d_fog, pvol_fog, nb * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        drk_scene_destroy(s);
+        return hip_fail(e, "drk_scene_create");
+    }
+    *out = s;
+    return DRK_OK;
+}
+
+int drk_scene_update_fog(drk_scene *s, const float *pvol_fog, const float *qval_fog) {
+    if (!s || !pvol_fog) return set_error(DRK_EINVAL, "drk_scene_update_fog: NULL argument");
+    size_t nb = (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc;
+    HIP_TRY(hipMemcpy(s->d_fog, pvol_fog, nb * sizeof(float), hipMemcpyDefault), "drk_scene_update_fog");
+    if (qval_fog) {
+        // qval_fog is indexed by node (host memory): patch every FOG majorant.
+        for (uint32_t i = 0; i < s->node_count; ++i) s->h_ob[i].qval[DRK_VOLUME_FOG] = qval_fog[i];
+        HIP_TRY(hipMemcpy(s->d_ob, s->h_ob.data(), sizeof(drk_octree_node) * s->node_count, hipMemcpyHostToDevice),
+                "drk_scene_update_fog");
+    }
+    return DRK_OK;
+}
+
+const drk_octree_node *drk_internal_scene_nodes(const drk_scene *s) { return s ? s->d_ob : nullptr; }
+
+void drk_scene_destroy(drk_scene *s) {
+    if (!s) return;
+    if (s->d_ob) (void)hipFree(s->d_ob);
+    if (s->d_fog) (void)hipFree(s->d_fog);
+    if (s->d_sdf) (void)hipFree(s->d_sdf);
+    delete s;
+}
+
+static void sampling_basis_host(const float iv[3], float b1[3], float b2[3]) {
+#pragma clang fp contract(off)
+    float len = std::sqrt(iv[0] * iv[0] + 0.0f * 0.0f + iv[2] * iv[2]);
+    b1[0] = -iv[2] / len;
+    b1[1] = 0.0f / len;
+    b1[2] = iv[0] / len;
+    b2[0] = iv[1] * b1[2] - iv[2] * b1[1];
+    b2[1] = iv[2] * b1[0] - iv[0] * b1[2];
+    b2[2] = iv[0] * b1[1] - iv[1] * b1[0];
+}
+
+int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float proji[16],
+                      const drk_sampler_desc *smp, const float *pdepth, uint32_t scattevs, float msigmas,
+                      float msigmaa, uint32_t tilex, uint32_t tiley, uint32_t w, uint32_t h, uint32_t flags,
+                      drk_kernel **out) {
+    if (!out) return set_error(DRK_EINVAL, "drk_kernel_create: out is NULL");
+    *out = nullptr;
+    if (!scene || !viewi || !proji || !smp) return set_error(DRK_EINVAL, "drk_kernel_create: NULL argument");
+    if (tilex == 0 || tiley == 0 || w == 0 || h == 0 || tilex > 16384 || tiley > 16384)
+        return set_error(DRK_EINVAL, "drk_kernel_create: bad tile/image size");
+    if (smp->light_count == 0 || smp->light_count > DRK_MAX_LIGHTS || !smp->lights)
+        return set_error(DRK_EINVAL, "drk_kernel_create: need 1..%d lights (lights[0] is sampled)", DRK_MAX_LIGHTS);
+    if (smp->phase != DRK_PHASE_MIE && smp->phase != DRK_PHASE_HG)
+        return set_error(DRK_EINVAL, "drk_kernel_create: unknown phase function %u", smp->phase);
+    if (smp->phase == DRK_PHASE_HG && (smp->hg_g == 0.0f || !(std::fabs(smp->hg_g) < 1.0f)))
+        return set_error(DRK_EINVAL, "drk_kernel_create: HG g must be in (-1,1) and non-zero");
+    if (smp->env > DRK_ENV_MAP) return set_error(DRK_EINVAL, "drk_kernel_create: unknown environment %u", smp->env);
+    if (smp->env == DRK_ENV_MAP && (!smp->env_map || smp->env_w < 2 || smp->env_h < 2))
+        return set_error(DRK_EINVAL, "drk_kernel_create: env map needs >= 2x2 texels");
+    if (smp->env == DRK_ENV_HOSEK && !(smp->sky_turbidity >= 1.0f && smp->sky_turbidity <= 10.0f &&
+                                      smp->sky_albedo >= 0.0f && smp->sky_albedo <= 1.0f))
+        return set_error(DRK_EINVAL, "drk_kernel_create: Hosek turbidity in [1,10], albedo in [0,1]");
+    if (!(msigmas >= 0.0f) || !(msigmaa >= 0.0f) || !(msigmas + msigmaa > 0.0f))
+        return set_error(DRK_EINVAL, "drk_kernel_create: cross sections must be >= 0 with a positive sum");
+    for (int i = 0; i < 16; ++i)
+        if (!std::isfinite(viewi[i]) || !std::isfinite(proji[i]))
+            return set_error(DRK_EINVAL, "drk_kernel_create: non-finite camera matrix");
+
+    drk_kernel *k = new (std::nothrow) drk_kernel;
+    if (!k) return set_error(DRK_ENOMEM, "drk_kernel_create: host allocation");
+    std::memset(&k->P, 0, sizeof(k->P));
+    RenderParams &P = k->P;
+    k->scene = scene;
+    k->tilex = tilex;
+    k->tiley = tiley;
+    P.ob = scene->d_ob;
+    P.pvol_fog = scene->d_fog;
+    P.pvol_sdf = scene->d_sdf;
+    P.lvoxc = scene->lvoxc;
+    P.lvoxc3 = scene->lvoxc * scene->lvoxc * scene->lvoxc;
+    std::memcpy(P.viewi, viewi, sizeof(P.viewi));
+    std::memcpy(P.proji, proji, sizeof(P.proji));
+    P.w = w;
+    P.h = h;
+    P.scattevs = scattevs;
+    P.sigma_s = msigmas;
+    P.sigma_e = msigmaa + msigmas;
+    P.phase = smp->phase;
+    P.hg_g = smp->hg_g;
+    P.env = smp->env;
+    P.light_count = smp->light_count;
+    for (uint32_t l = 0; l < smp->light_count; ++l) {
+        for (int c = 0; c < 3; ++c) {
+            P.sun_dir[l][c] = smp->lights[l].direction[c];
+            P.sun_color[l][c] = smp->lights[l].color[c];
+        }
+        P.sun_cos[l] = cosf(smp->lights[l].angle);   // SunLight ctor, KernelSampler.cpp:306
+    }
+    {
+#pragma clang fp contract(off)
+        float ctm = P.sun_cos[0];
+        P.sun_pdf0 = 1.0f / (2.0f * kPi * (1.0f - ctm * ctm));
+    }
+    sampling_basis_host(P.sun_dir[0], P.sun_b1, P.sun_b2);
+    if (smp->env == DRK_ENV_HOSEK) {
+        // RenderKernel::Initialize, src/kernel.cpp:521-526, with the turbidity
+        // and albedo made parameters.
+        float th = acosf(P.sun_dir[0][2]);
+        float se = 1.57079632679489661923f - th;
+        double cfg[3][9], rad[3];
+        hosek_state((double)smp->sky_turbidity, (double)smp->sky_albedo, (double)se, cfg, rad);
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 9; ++j) P.sky.cfg[i][j] = (float)cfg[i][j];
+            P.sky.rad[i] = (float)rad[i];
+        }
+        for (int c = 0; c < 3; ++c) P.sky.dir[c] = P.sun_dir[0][c];
+    }
+
+    hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
+    size_t tile_bytes = (size_t)tilex * tiley * 16;
+    if (e == hipSuccess) e = hipMalloc(&k->d_out, 4 * tile_bytes);
+    if (e == hipSuccess) e = hipMalloc(&k->d_counter, 256);
+    if (e == hipSuccess) e = hipMalloc(&k->d_counts, 64);
+    if (e == hipSuccess && smp->env == DRK_ENV_MAP) {
+        size_t eb = (size_t)smp->env_w * smp->env_h * 16;
+        e = hipMalloc(&k->d_env, eb);
+        if (e == hipSuccess) e = hipMemcpy(k->d_env, smp->env_map, eb, hipMemcpyHostToDevice);
+        P.env_map = k->d_env;
+        P.env_w = smp->env_w;
+        P.env_h = smp->env_h;
+    }
+    if (e == hipSuccess && pdepth && (flags & DRK_KERNEL_DEPTHCOMP)) {
+        e = hipMalloc(&k->d_depth, (size_t)w * h * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(k->d_depth, pdepth, (size_t)w * h * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        drk_kernel_destroy(k);
+        return hip_fail(e, "drk_kernel_create");
+    }
+    k->h_phb = static_cast<float *>(std::aligned_alloc(64, ((4 * tile_bytes + 63) / 64) * 64));
+    if (!k->h_phb) {
+        drk_kernel_destroy(k);
+        return set_error(DRK_ENOMEM, "drk_kernel_create: host result buffers");
+    }
+    std::memset(k->h_phb, 0, 4 * tile_bytes);
+    uint32_t cus = 0;
+    if (drk_device_info(nullptr, 0, &cus) != DRK_OK || cus == 0) cus = 1;
+    k->cu_count = cus;
+    k->blocks_per_cu = (uint32_t)render_blocks_per_cu(true, false);
+    if (const char *c = std::getenv("DRK_CHUNK")) k->chunk_override = (uint32_t)std::atoi(c);
+    if (const char *g = std::getenv("DRK_GRID")) k->grid_override = (uint32_t)std::atoi(g);
+    *out = k;
+    return DRK_OK;
+}
+
+int drk_kernel_set_seed(drk_kernel *k, uint64_t seed, uint32_t sample_base) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_seed: NULL kernel");
+    k->seed = seed;
+    k->sample_base = sample_base;
+    return DRK_OK;
+}
+
+int drk_kernel_set_tuning(drk_kernel *k, uint32_t chunk, uint32_t grid_blocks) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_tuning: NULL kernel");
+    k->chunk_override = chunk;
+    k->grid_override = grid_blocks;
+    return DRK_OK;
+}
+
+static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+                       uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
+                       hipStream_t s, bool count) {
+    if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
+        return set_error(DRK_EINVAL, "render: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley, k->tilex, k->tiley);
+    if (samples == 0) return set_error(DRK_EINVAL, "render: samples must be > 0");
+    if ((uint64_t)x0 + tilex > 0xffffffffull || (uint64_t)y0 + tiley > 0xffffffffull)
+        return set_error(DRK_EINVAL, "render: tile origin overflows");
+    if ((uint64_t)sample_base + samples > 0xffffffffull) return set_error(DRK_EINVAL, "render: sample counter overflows");
+    RenderParams P = k->P;
+    P.x0 = x0;
+    P.y0 = y0;
+    P.rx = tilex & ~1u;   // K_ParallelRender iterates rx/2 packets of 2 (src/kernel.cpp:453)
+    P.ry = tiley & ~1u;
+    P.tilex = tilex;
+    P.samples = samples;
+    P.sample_base = sample_base;
+    P.key1 = rng_key1(seed);
+    P.depth = k->d_depth;
+    const bool sumsq = d_sql && d_sqe;
+    // Pixels outside the rendered extent (odd tiles) or outside the image keep
+    // zeros; the reference leaves them untouched.
+    size_t tile_n = (size_t)tilex * tiley * 4;
+    HIP_TRY(launch_fill(d_l, tile_n, 0.0f, s), "render: clear");
+    HIP_TRY(launch_fill(d_e, tile_n, 0.0f, s), "render: clear");
+    if (sumsq) {
+        HIP_TRY(launch_fill(d_sql, tile_n, 0.0f, s), "render: clear");
+        HIP_TRY(launch_fill(d_sqe, tile_n, 0.0f, s), "render: clear");
+    }
+    // Clip the rendered extent to the image (gm mask, src/kernel.cpp:472-474).
+    if (P.x0 >= P.w || P.y0 >= P.h) return DRK_OK;
+    uint32_t rx = std::min(P.rx, P.w - P.x0), ry = std::min(P.ry, P.h - P.y0);
+    P.rx = rx;
+    P.ry = ry;
+    if (rx == 0 || ry == 0) return DRK_OK;
+
+    const uint32_t npix = rx * ry;
+    const uint32_t bpc = count ? (uint32_t)render_blocks_per_cu(true, true) : k->blocks_per_cu;
+    uint32_t grid = k->grid_override ? k->grid_override : k->cu_count * std::max(1u, bpc);
+    const uint64_t lanes = (uint64_t)grid * render_block_size();
+    uint32_t chunk = k->chunk_override;
+    if (chunk == 0) {
+        // Aim for >= 16 units per lane so the tail of the drain stays short.
+        uint64_t want_units = lanes * 16;
+        uint64_t nch = (want_units + npix - 1) / npix;
+        nch = std::max<uint64_t>(1, std::min<uint64_t>(nch, samples));
+        chunk = (uint32_t)((samples + nch - 1) / nch);
+    }
+    chunk = std::max(1u, std::min(chunk, samples));
+    const uint32_t nchunks = (samples + chunk - 1) / chunk;
+    if ((uint64_t)npix * nchunks > 0xffffffffull) return set_error(DRK_EINVAL, "render: too many work units");
+    WorkParams W;
+    std::memset(&W, 0, sizeof(W));
+    W.counter = k->d_counter;
+    W.npix = npix;
+    W.chunk = chunk;
+    W.nchunks = nchunks;
+    W.total_units = npix * nchunks;
+    W.out_l = d_l;
+    W.out_e = d_e;
+    W.sq_l = sumsq ? d_sql : nullptr;
+    W.sq_e = sumsq ? d_sqe : nullptr;
+    W.counts = k->d_counts;
+    if (nchunks > 1) {
+        size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
+        if (need > k->partial_bytes) {
+            if (k->d_partial) (void)hipFree(k->d_partial);
+            k->d_partial = nullptr;
+            k->partial_bytes = 0;
+            HIP_TRY(hipMalloc(&k->d_partial, need), "render: partial sums");
+            k->partial_bytes = need;
+        }
+        W.partial = k->d_partial;
+    }
+    if (count) HIP_TRY(hipMemsetAsync(k->d_counts, 0, 16, s), "render: counters");
+    HIP_TRY(launch_render(P, W, grid, sumsq, count, s), "render: launch");
+    return DRK_OK;
+}
+
+int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                             uint32_t samples, uint64_t seed, uint32_t sample_base, float *d_lights, float *d_env,
+                             float *d_sumsq_lights, float *d_sumsq_env, void *stream) {
+    if (!k || !d_lights || !d_env) return set_error(DRK_EINVAL, "drk_kernel_render_device: NULL argument");
+    if ((d_sumsq_lights == nullptr) != (d_sumsq_env == nullptr))
+        return set_error(DRK_EINVAL, "drk_kernel_render_device: pass both sum-of-squares buffers or neither");
+    hipStream_t s = stream ? (hipStream_t)stream : k->stream;
+    return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, d_sumsq_lights,
+                       d_sumsq_env, s, false);
+}
+
+int drk_kernel_render(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_render: NULL kernel");
+    size_t tb = (size_t)k->tilex * k->tiley * 16;
+    char *base = reinterpret_cast<char *>(k->d_out);
+    float *b[4] = {(float *)base, (float *)(base + tb), (float *)(base + 2 * tb), (float *)(base + 3 * tb)};
+    int rc = render_impl(k, x0, y0, tilex, tiley, samples, k->seed, k->sample_base, b[0], b[1], b[2], b[3],
+                         k->stream, false);
+    if (rc != DRK_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(k->h_phb, k->d_out, 4 * tb, hipMemcpyDeviceToHost, k->stream), "drk_kernel_render: copy-out");
+    HIP_TRY(hipStreamSynchronize(k->stream), "drk_kernel_render: sync");
+    k->sample_base += samples;
+    return DRK_OK;
+}
+
+// Counting pass for the algorithmic-bytes model: same launch with per-lane
+// counters of node-record reads and brick lookups (identical RNG keys, so the
+// counts describe the timed run exactly).
+int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+                     uint64_t seed, uint32_t sample_base, unsigned long long counts[2]) {
+    if (!k || !counts) return set_error(DRK_EINVAL, "drk_kernel_count: NULL argument");
+    size_t tb = (size_t)k->tilex * k->tiley * 16;
+    char *base = reinterpret_cast<char *>(k->d_out);
+    int rc = render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, (float *)base, (float *)(base + tb),
+                         (float *)(base + 2 * tb), (float *)(base + 3 * tb), k->stream, true);
+    if (rc != DRK_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(counts, k->d_counts, 16, hipMemcpyDeviceToHost, k->stream), "drk_kernel_count: copy");
+    HIP_TRY(hipStreamSynchronize(k->stream), "drk_kernel_count: sync");
+    return DRK_OK;
+}
+
+int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples) {
+    (void)x0; (void)y0; (void)tilex; (void)tiley; (void)samples;
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_shadow: NULL kernel");
+    return set_error(DRK_EINVAL, "drk_kernel_shadow: the Shadow pass is not implemented yet");
+}
+
+const float *drk_kernel_phb(const drk_kernel *k, uint32_t buffer) {
+    if (!k || buffer > 3) {
+        set_error(DRK_EINVAL, "drk_kernel_phb: bad kernel or buffer index");
+        return nullptr;
+    }
+    return k->h_phb + (size_t)buffer * k->tilex * k->tiley * 4;
+}
+
+void drk_kernel_destroy(drk_kernel *k) {
+    if (!k) return;
+    if (k->stream) (void)hipStreamSynchronize(k->stream);
+    if (k->d_out) (void)hipFree(k->d_out);
+    if (k->d_partial) (void)hipFree(k->d_partial);
+    if (k->d_counter) (void)hipFree(k->d_counter);
+    if (k->d_counts) (void)hipFree(k->d_counts);
+    if (k->d_env) (void)hipFree(k->d_env);
+    if (k->d_depth) (void)hipFree(k->d_depth);
+    if (k->stream) (void)hipStreamDestroy(k->stream);
+    std::free(k->h_phb);
+    delete k;
+}
+
+}  // extern "C"

@@ -1,0 +1,436 @@
+// Device-side pieces of droplet-render's volumetric estimator, restated for
+// gfx950: camera rays, Revelles octree traversal, trilinear brick lookup,
+// Mie / Henyey-Greenstein phase, sun cone, Hosek sky and equirect env.
+// Each function cites the reference code whose behaviour it reproduces.
+#pragma once
+
+#include "drk_common.h"
+
+namespace drk {
+
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 mk3(float x, float y, float z) { return {x, y, z}; }
+
+// Mie tables (data, see data/mie_*.inc), one copy per translation unit.
+static __constant__ float c_mie_pdf[kMieBins * 3] = {
+#include "../data/mie_pdf_rgb.inc"
+};
+static __constant__ float c_mie_cdf_rgb[kMieBins * 3] = {
+#include "../data/mie_cdf_rgb.inc"
+};
+
+// Tables staged in LDS once per workgroup.
+struct MieLds {
+    float pdf[kMieBins * 3];   // miepdf RGB, KernelSampler.cpp:8-73
+    float cdf[kMieBins];       // miecdf R channel, the only one Sample reads (KernelSampler.cpp:262-263)
+};
+
+__device__ __forceinline__ void load_mie(MieLds &m) {
+    for (uint32_t i = threadIdx.x; i < kMieBins * 3; i += blockDim.x) m.pdf[i] = c_mie_pdf[i];
+    for (uint32_t i = threadIdx.x; i < kMieBins; i += blockDim.x) m.cdf[i] = c_mie_cdf_rgb[i * 3];
+}
+
+// ---------------------------------------------------------------------------
+// xoshiro128** (Blackman & Vigna) per-ray stream, seeded from Philox so that
+// every (pixel, sample, ray) has a replayable sequence of uniforms.
+// ---------------------------------------------------------------------------
+struct RayRng {
+    uint32_t s0, s1, s2, s3;
+    __device__ __forceinline__ void seed(uint32_t key0, uint32_t key1, uint32_t sample, uint32_t ray) {
+        U4 v = philox4x32_10(U4{sample, ray, 0u, kRngTrack}, key0, key1);
+        s0 = v.x; s1 = v.y; s2 = v.z; s3 = v.w | ((v.x | v.y | v.z) == 0u ? 1u : 0u);
+    }
+    __device__ __forceinline__ uint32_t next() {
+        uint32_t r = s1 * 5u;
+        r = ((r << 7) | (r >> 25)) * 9u;
+        uint32_t t = s1 << 9;
+        s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t;
+        s3 = (s3 << 11) | (s3 >> 21);
+        return r;
+    }
+    __device__ __forceinline__ float uniform() { return u01(next()); }
+};
+
+// ---------------------------------------------------------------------------
+// Primary ray, K_ParallelRender (src/kernel.cpp:461-474): NDC with the
+// reference's sign flips, ro = viewi.r[3], rd = normalize((posh*proji).xyz0 * viewi).
+// Row-vector convention: v*M = sum_k v[k]*M.r[k].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void camera_ray(const RenderParams &P, uint32_t px, uint32_t py, F3 &ro, F3 &rd, bool &gm) {
+#pragma clang fp contract(off)
+    float hx = -(2.0f * ((float)px + 0.5f) / (float)P.w - 1.0f);
+    float hy = -(2.0f * ((float)py + 0.5f) / (float)P.h - 1.0f);
+    const float *pi = P.proji;
+    float rv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) rv[j] = hx * pi[0 * 4 + j] + hy * pi[1 * 4 + j] + 0.0f * pi[2 * 4 + j] + 1.0f * pi[3 * 4 + j];
+    const float *vi = P.viewi;
+    float d[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[j] = rv[0] * vi[0 * 4 + j] + rv[1] * vi[1 * 4 + j] + rv[2] * vi[2 * 4 + j] + 0.0f * vi[3 * 4 + j];
+    float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    rd = mk3(d[0] / len, d[1] / len, d[2] / len);
+    ro = mk3(vi[12], vi[13], vi[14]);
+    gm = hx > -1.0f && hx < 1.0f && hy > -1.0f && hy < 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Revelles et al. parametric octree traversal (KernelOctree.cpp:9-110 and the
+// step traverser :222-349). Per-lane stack lives in LDS, one 32-bit word per
+// level: node index (bits 0-23), next child tn (bits 24-27), child being
+// visited c (bits 28-30). The t-intervals of a level are recomputed from the
+// root interval along the stored child path with the reference's exact
+// operations (tm = 0.5f*(t0+t1)), so they are bit-identical to the values the
+// reference keeps on its stack.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t first_node(F3 t0, F3 tm) {
+    uint32_t a = 0;
+    if (t0.x > t0.y) {
+        if (t0.x > t0.z) {
+            if (tm.y < t0.x) a |= 2;
+            if (tm.z < t0.x) a |= 1;
+            return a;
+        }
+    } else {
+        if (t0.y > t0.z) {
+            if (tm.x < t0.y) a |= 4;
+            if (tm.z < t0.y) a |= 1;
+            return a;
+        }
+    }
+    if (tm.x < t0.z) a |= 4;
+    if (tm.y < t0.z) a |= 2;
+    return a;
+}
+
+// Exit plane of child c whose far corner is t1: x -> c|4, y -> c|2, z -> c|1,
+// or 8 when that bit is already set (the successor table at KernelOctree.cpp:165-196,
+// with OctreeNextNode's tie order :62-71).
+__device__ __forceinline__ uint32_t next_node(F3 t1, uint32_t c) {
+    uint32_t nx = (c & 4) ? 8u : (c | 4), ny = (c & 2) ? 8u : (c | 2), nz = (c & 1) ? 8u : (c | 1);
+    if (t1.x < t1.y) {
+        if (t1.x < t1.z) return nx;
+    } else {
+        if (t1.y < t1.z) return ny;
+    }
+    return nz;
+}
+
+__device__ __forceinline__ void child_interval(F3 t0, F3 t1, uint32_t c, F3 &ct0, F3 &ct1) {
+#pragma clang fp contract(off)
+    F3 tm = mk3(0.5f * (t0.x + t1.x), 0.5f * (t0.y + t1.y), 0.5f * (t0.z + t1.z));
+    ct0.x = (c & 4) ? tm.x : t0.x; ct1.x = (c & 4) ? t1.x : tm.x;
+    ct0.y = (c & 2) ? tm.y : t0.y; ct1.y = (c & 2) ? t1.y : tm.y;
+    ct0.z = (c & 1) ? tm.z : t0.z; ct1.z = (c & 1) ? t1.z : tm.z;
+}
+
+__device__ __forceinline__ bool is_leaf(const drk_octree_node &n) {
+    return n.volx[DRK_VOLUME_SDF] != DRK_NO_BRICK || n.volx[DRK_VOLUME_FOG] != DRK_NO_BRICK;
+}
+
+// OctreeInitialize (KernelOctree.cpp:78-110): mirror to a positive direction
+// in the root-local frame; returns false when the ray misses the root box
+// (the max(t0) < min(t1) test of :137/:238).
+__device__ __forceinline__ bool octree_init(const drk_octree_node *ob, F3 ro, F3 rd, F3 &t0, F3 &t1, uint32_t &a) {
+#pragma clang fp contract(off)
+    float4 ce = *reinterpret_cast<const float4 *>(ob[0].ce);
+    float e = ce.w;
+    F3 o = mk3((ro.x - ce.x) + e, (ro.y - ce.y) + e, (ro.z - ce.z) + e);
+    F3 d = rd;
+    a = 0;
+    if (d.x < 0.0f) { o.x = 2.0f * e - o.x; d.x = -d.x; a |= 4; }
+    if (d.y < 0.0f) { o.y = 2.0f * e - o.y; d.y = -d.y; a |= 2; }
+    if (d.z < 0.0f) { o.z = 2.0f * e - o.z; d.z = -d.z; a |= 1; }
+    float e2 = 2.0f * e;
+    t0 = mk3((0.0f - o.x) / d.x, (0.0f - o.y) / d.y, (0.0f - o.z) / d.z);
+    t1 = mk3((e2 - o.x) / d.x, (e2 - o.y) / d.y, (e2 - o.z) / d.z);
+    return fmaxf(fmaxf(t0.x, t0.y), t0.z) < fminf(fminf(t1.x, t1.y), t1.z);
+}
+
+// Child order slot map nla = {0,4,2,6,1,5,3,7} (KernelOctree.cpp:164): Revelles
+// index (x=4,y=2,z=1) -> build slot (x=1,y=2,z=4), a 3-bit reversal.
+__device__ __forceinline__ uint32_t build_slot(uint32_t r) { return ((r & 4) >> 2) | (r & 2) | ((r & 1) << 2); }
+
+struct Traversal {
+    F3 rt0, rt1;      // root interval
+    F3 t0, t1;        // interval of the node on top of the stack
+    uint32_t a;       // mirror flags
+    int lvl;          // top of stack, -1 = exhausted
+};
+
+constexpr uint32_t kStkNodeMask = 0x00ffffffu;
+
+// Advances to the next leaf in ray order. Returns false when exhausted.
+// `touches` counts node-record reads (for the algorithmic-bytes model).
+__device__ __forceinline__ bool trav_next(const drk_octree_node *ob, Traversal &T, uint32_t *stk, uint32_t stride,
+                                          uint32_t &leaf, float &tra, float &trb, uint32_t &touches) {
+    for (;;) {
+        if (T.lvl < 0) return false;
+        uint32_t f = stk[T.lvl * stride];
+        uint32_t tn = (f >> 24) & 15u;
+        if (tn >= 8) {
+            // Pop; recompute the parent's interval from the root along the path.
+            if (--T.lvl < 0) return false;
+            F3 a0 = T.rt0, a1 = T.rt1;
+            for (int k = 0; k < T.lvl; ++k) {
+                uint32_t ck = (stk[k * stride] >> 28) & 7u;
+                F3 b0, b1;
+                child_interval(a0, a1, ck, b0, b1);
+                a0 = b0; a1 = b1;
+            }
+            T.t0 = a0; T.t1 = a1;
+            continue;
+        }
+        uint32_t node = f & kStkNodeMask;
+        F3 ct0, ct1;
+        child_interval(T.t0, T.t1, tn, ct0, ct1);
+        uint32_t nxt = (tn == 7) ? 8u : next_node(ct1, tn);
+        stk[T.lvl * stride] = node | (nxt << 24) | (tn << 28);
+        uint32_t ch = ob[node].chn[build_slot(tn ^ T.a)];
+        touches += 1;
+        if (ct1.x < 0.0f || ct1.y < 0.0f || ct1.z < 0.0f || ch == 0) continue;
+        touches += 1;
+        if (is_leaf(ob[ch])) {
+            leaf = ch;
+            tra = fmaxf(fmaxf(ct0.x, ct0.y), ct0.z);
+            trb = fminf(fminf(ct1.x, ct1.y), ct1.z);
+            return true;
+        }
+        // Interior child: push.
+        F3 tm;
+        {
+#pragma clang fp contract(off)
+            tm = mk3(0.5f * (ct0.x + ct1.x), 0.5f * (ct0.y + ct1.y), 0.5f * (ct0.z + ct1.z));
+        }
+        ++T.lvl;
+        stk[T.lvl * stride] = ch | (first_node(ct0, tm) << 24);
+        T.t0 = ct0; T.t1 = ct1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SampleVoxelSpace (src/kernel.cpp:96-126): trilinear lookup in one
+// lvoxc^3 vertex-centred brick, lerping x, then y, then z; lerp(a,b,t) is
+// t*(b-a)+a with separate rounding, as the reference's SSE3 build computes it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float brick_lookup(const float *__restrict__ brick, F3 p, float4 ce, uint32_t lvoxc) {
+#pragma clang fp contract(off)
+    const float nv = (float)lvoxc;
+    const float nvm = nv - 1.0f;
+    float ni[3] = {((-0.5f * ((ce.x - ce.w) - p.x)) * nvm) / ce.w,
+                   ((-0.5f * ((ce.y - ce.w) - p.y)) * nvm) / ce.w,
+                   ((-0.5f * ((ce.z - ce.w) - p.z)) * nvm) / ce.w};
+    uint32_t ia[3], ib[3];
+    float nl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float nf = fmaxf(floorf(ni[k]), 0.0f);
+        float nc = fminf(ceilf(ni[k]), nvm);
+        nl[k] = ni[k] - nf;
+        nc = fmaxf(fminf(nc, nvm), 0.0f);
+        nf = fmaxf(fminf(nf, nvm), 0.0f);
+        ia[k] = (uint32_t)nf;
+        ib[k] = (uint32_t)nc;
+    }
+    const uint32_t l2 = lvoxc * lvoxc;
+    const uint32_t za = ia[2] * l2, zb = ib[2] * l2, ya = ia[1] * lvoxc, yb = ib[1] * lvoxc;
+    float a00 = brick[za + ya + ia[0]], a10 = brick[za + yb + ia[0]], a01 = brick[zb + ya + ia[0]], a11 = brick[zb + yb + ia[0]];
+    float b00 = brick[za + ya + ib[0]], b10 = brick[za + yb + ib[0]], b01 = brick[zb + ya + ib[0]], b11 = brick[zb + yb + ib[0]];
+    float u00 = nl[0] * (b00 - a00) + a00;
+    float u10 = nl[0] * (b10 - a10) + a10;
+    float u01 = nl[0] * (b01 - a01) + a01;
+    float u11 = nl[0] * (b11 - a11) + a11;
+    float v0 = nl[1] * (u10 - u00) + u00;
+    float v1 = nl[1] * (u11 - u01) + u01;
+    return nl[2] * (v1 - v0) + v0;
+}
+
+// ---------------------------------------------------------------------------
+// Phase functions.
+// ---------------------------------------------------------------------------
+// SamplingBasis (KernelSampler.cpp:145-153); degenerate for iv = +-y like the reference.
+__device__ __forceinline__ void sampling_basis(F3 iv, F3 &b1, F3 &b2) {
+#pragma clang fp contract(off)
+    float len = sqrtf(iv.x * iv.x + 0.0f * 0.0f + iv.z * iv.z);
+    b1 = mk3(-iv.z / len, 0.0f / len, iv.x / len);
+    b2 = mk3(iv.y * b1.z - iv.z * b1.y, iv.z * b1.x - iv.x * b1.z, iv.x * b1.y - iv.y * b1.x);
+}
+
+__device__ __forceinline__ F3 basis_dir(F3 b1, F3 b2, F3 iv, float st, float ct, float ph) {
+    float sph, cph;
+    sincosf(ph, &sph, &cph);
+    return mk3(b1.x * st * cph + b2.x * st * sph + iv.x * ct,
+               b1.y * st * cph + b2.y * st * sph + iv.y * ct,
+               b1.z * st * cph + b2.z * st * sph + iv.z * ct);
+}
+
+// MiePhase::EvaluateRGB (KernelSampler.cpp:233-256): table lerp on
+// a = acos(-ct)/pi with ct clamped to [-1,1]. Uses an exact acos where the
+// reference uses a 4-term polynomial approximation.
+__device__ __forceinline__ F3 mie_eval_rgb(const MieLds &T, float ct) {
+#pragma clang fp contract(off)
+    float ct1 = fmaxf(fminf(ct, 1.0f), -1.0f);
+    float a = acosf(-ct1) / kPi;
+    float b = 256.0f * a;
+    float c = fmaxf(fminf(floorf(b), 254.0f), 0.0f);
+    float t = b - c;
+    uint32_t i = (uint32_t)c * 3u;
+    return mk3(t * (T.pdf[i + 3] - T.pdf[i + 0]) + T.pdf[i + 0],
+               t * (T.pdf[i + 4] - T.pdf[i + 1]) + T.pdf[i + 1],
+               t * (T.pdf[i + 5] - T.pdf[i + 2]) + T.pdf[i + 2]);
+}
+
+// Key k_i of MiePhase::Sample's scan (KernelSampler.cpp:261-271) and the
+// angle it would assign: b = max(ub,i), c = min(floor b,255), t = b-c.
+__device__ __forceinline__ float mie_key(const MieLds &T, float ub, uint32_t i, float &t) {
+#pragma clang fp contract(off)
+    float b = fmaxf(ub, (float)i);
+    float c = fminf(floorf(b), 255.0f);
+    t = b - c;
+    float c0 = T.cdf[i], c1 = T.cdf[i + 1];
+    return t * (c1 - c0) + c0;
+}
+
+// MiePhase::Sample (KernelSampler.cpp:258-285). The reference scans
+// i = 0..254 and keeps the angle of the last i whose key is below u1; the keys
+// are non-decreasing in i, so a binary search for the first key >= u1 gives
+// the same i.
+__device__ __forceinline__ F3 mie_sample(const MieLds &T, F3 iv, float u1, float u2) {
+#pragma clang fp contract(off)
+    float ub = 256.0f * u1;
+    uint32_t lo = 0, hi = 255;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        float t;
+        if (mie_key(T, ub, mid, t) < u1) lo = mid + 1; else hi = mid;
+    }
+    float th = 0.0f;
+    if (lo > 0) {
+        float t;
+        uint32_t i = lo - 1;
+        (void)mie_key(T, ub, i, t);
+        th = (float)i / 256.0f + t / 256.0f;
+    }
+    th *= kPi;
+    float ph = (2.0f * kPi) * u2;
+    F3 b1, b2;
+    sampling_basis(iv, b1, b2);
+    float ct = -cosf(th);
+    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+    return basis_dir(b1, b2, iv, st, ct, ph);
+}
+
+// HGPhase (KernelSampler.cpp:165-204).
+__device__ __forceinline__ float hg_eval(float g, float ct) {
+    return (1.0f - g * g) / (4.0f * kPi * powf(1.0f + g * g - 2.0f * g * ct, 1.5f));
+}
+__device__ __forceinline__ F3 hg_sample(float g, F3 iv, float u1, float u2) {
+    float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+    float ct = (1.0f + g * g - sq * sq) / (2.0f * g);
+    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+    float ph = (2.0f * kPi) * u2;
+    F3 b1, b2;
+    sampling_basis(iv, b1, b2);
+    return basis_dir(b1, b2, iv, st, ct, ph);
+}
+
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// ---------------------------------------------------------------------------
+// Sky: Hosek-Wilkie radiance with the reference's gamma construction and
+// exposure (src/kernel.cpp:329-360).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ F3 hosek_sky(const SkyState &S, F3 rd) {
+    float rdz = rd.z;
+    float sth = sqrtf(1.0f - rd.z * rd.z);
+    float cth = rdz;
+    float slth = sqrtf(1.0f - S.dir[2] * S.dir[2]);
+    float clth = S.dir[2];
+    float slph = S.dir[1];
+    float clph = S.dir[0];
+    float rdd = rd.x / rd.y;
+    float acr = 1.0f / sqrtf(rdd * rdd + 1.0f);
+    float czp = acr * (rdd * slph + clph);
+    float cph = sth * slth * czp + cth * clth;
+    float gmma = acosf(cph);
+    float gacs = fmaxf(fminf(cph, 1.0f), 0.0f);
+    float thcs = fmaxf(rdz, 0.0f);
+    float raym = gacs * gacs;
+    float zenh = sqrtf(thcs);
+    float out[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float *c = S.cfg[i];
+        float expm = expf(c[4] * gmma);
+        float miem = (1.0f + raym) / powf(1.0f + c[8] * c[8] - 2.0f * c[8] * gacs, 1.5f);
+        float ca = (1.0f + c[0] * expf(c[1] / (thcs + 0.01f))) * (c[2] + c[3] * expm + c[5] * raym + c[6] * miem + c[7] * zenh);
+        ca *= S.rad[i];
+        out[i] = 1e-3f * powf(ca, 2.2f);
+    }
+    return mk3(out[0], out[1], out[2]);
+}
+
+// MapEnv::Evaluate (KernelSampler.cpp:369-421), followed by the pow(.,2.2)
+// of src/kernel.cpp:362-364.
+__device__ __forceinline__ F3 map_env(const float *__restrict__ tex, uint32_t w, uint32_t h, F3 rd) {
+    float tha = acosf(-rd.z) / kPi;
+    float thb = (float)h * fmaxf(fminf(tha, 1.0f), 0.0f);
+    float thc = fmaxf(fminf(floorf(thb), (float)(h - 2)), 0.0f);
+    float tht = thb - thc;
+    float pha = 0.5f + atan2f(rd.y, -rd.x) / (2.0f * kPi);
+    float phb = (float)w * fmaxf(fminf(pha, 1.0f), 0.0f);
+    float phc = fmaxf(fminf(floorf(phb), (float)(w - 2)), 0.0f);
+    float pht = phb - phc;
+    uint32_t ti = (uint32_t)thc, pj = (uint32_t)phc;
+    float out[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float ca = tex[(ti * w + pj) * 4 + i];
+        float cb = tex[((ti + 1) * w + pj) * 4 + i];
+        float da = tex[(ti * w + pj + 1) * 4 + i];
+        float db = tex[((ti + 1) * w + pj + 1) * 4 + i];
+        float l0 = tht * (cb - ca) + ca;
+        float l1 = tht * (db - da) + da;
+        out[i] = powf(pht * (l1 - l0) + l0, 2.2f);
+    }
+    return mk3(out[0], out[1], out[2]);
+}
+
+// Environment radiance for an escaping continuation ray.
+__device__ __forceinline__ F3 env_radiance(const RenderParams &P, F3 rd) {
+    if (P.env == DRK_ENV_HOSEK) return hosek_sky(P.sky, rd);
+    if (P.env == DRK_ENV_MAP) return map_env(P.env_map, P.env_w, P.env_h, rd);
+    return mk3(0.0f, 0.0f, 0.0f);   // NullEnv: pow(0,2.2) = 0
+}
+
+// Sum of SunLight::Evaluate over all lights (src/kernel.cpp:326-327).
+__device__ __forceinline__ F3 lights_eval(const RenderParams &P, F3 rd) {
+    F3 s = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t l = 0; l < DRK_MAX_LIGHTS; ++l) {
+        if (l >= P.light_count) break;
+        float d = rd.x * P.sun_dir[l][0] + rd.y * P.sun_dir[l][1] + rd.z * P.sun_dir[l][2];
+        if (d > P.sun_cos[l]) { s.x += P.sun_color[l][0]; s.y += P.sun_color[l][1]; s.z += P.sun_color[l][2]; }
+    }
+    return s;
+}
+
+// SunLight::Pdf of lights[0] (KernelSampler.cpp:326-330).
+__device__ __forceinline__ float sun_pdf(const RenderParams &P, F3 v) {
+    float d = v.x * P.sun_dir[0][0] + v.y * P.sun_dir[0][1] + v.z * P.sun_dir[0][2];
+    return d > P.sun_cos[0] ? P.sun_pdf0 : 0.0f;
+}
+
+// SunLight::Sample of lights[0] (KernelSampler.cpp:332-345): cos uniform in [cos angle, 1].
+__device__ __forceinline__ F3 sun_sample(const RenderParams &P, float u1, float u2) {
+    float ctm = P.sun_cos[0];
+    float ct = (1.0f - u1) + u1 * ctm;
+    float st = sqrtf(1.0f - ct * ct);
+    float ph = (2.0f * kPi) * u2;
+    F3 D = mk3(P.sun_dir[0][0], P.sun_dir[0][1], P.sun_dir[0][2]);
+    F3 b1 = mk3(P.sun_b1[0], P.sun_b1[1], P.sun_b1[2]);
+    F3 b2 = mk3(P.sun_b2[0], P.sun_b2[1], P.sun_b2[2]);
+    return basis_dir(b1, b2, D, st, ct, ph);
+}
+
+}  // namespace drk

@@ -1,0 +1,30 @@
+// Launch interface between the host library and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "drk_common.h"
+
+namespace drk {
+
+// Work decomposition of one render launch: units are (pixel, sample chunk)
+// pairs, unit u -> chunk u / npix, pixel u % npix (row-major in the tile).
+struct WorkParams {
+    uint32_t *counter;        // next unfetched unit (device, reset per launch)
+    uint32_t total_units;     // npix * nchunks
+    uint32_t npix;            // rx * ry
+    uint32_t chunk;           // samples per unit
+    uint32_t nchunks;
+    float *partial;           // [buffers][nchunks][npix] float4 when nchunks > 1, else nullptr
+    float *out_l, *out_e;     // tile float4 outputs (stride tilex)
+    float *sq_l, *sq_e;       // sum-of-squares outputs or nullptr
+    unsigned long long *counts;  // [node touches, brick lookups] for the counting pass
+};
+
+int render_block_size();
+int render_blocks_per_cu(bool sumsq, bool count);
+hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, bool sumsq, bool count,
+                         hipStream_t s);
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+
+}  // namespace drk

@@ -1,0 +1,108 @@
+"""RenderKernel: the tile renderer, on the MI355X.
+
+Python mirror of droplet-render's `RenderKernel` (src/kernel.h:11-59):
+Initialize / Render / Shadow / Destroy and the two result buffers `phb`
+(BUFFER_LIGHTS, BUFFER_ENVIRONMENT), each tilex*tiley RGBA sums. The work runs
+in libdroplet_amd.so's gfx950 kernels; this class only marshals arguments.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .sampler import sampler_desc
+
+
+class RenderKernel:
+    BUFFER_LIGHTS = 0
+    BUFFER_ENVIRONMENT = 1
+    BUFFER_COUNT = 2
+
+    def __init__(self):
+        self._h = None
+        self._keep = None
+        self.tilew = self.tileh = 0
+
+    def Initialize(self, scene, viewi, proji, ppf, penv, pdepth, scattevs, msigmas, msigmaa, tilex, tiley, w, h,
+                   flags=0, lights=()):
+        """RenderKernel::Initialize (src/kernel.cpp:497-534). `lights` replaces
+        the process-global BaseLight::lights registry; lights[0] is sampled."""
+        self.Destroy()
+        viewi = np.ascontiguousarray(viewi, dtype=np.float32).reshape(16)
+        proji = np.ascontiguousarray(proji, dtype=np.float32).reshape(16)
+        desc, keep = sampler_desc(ppf, penv, lights)
+        depth = None if pdepth is None else np.ascontiguousarray(pdepth, dtype=np.float32).reshape(-1)
+        hnd = C.c_void_p()
+        L.check(L.lib().drk_kernel_create(scene.device(), L.fptr(viewi), L.fptr(proji), C.byref(desc), L.fptr(depth),
+                                          int(scattevs), float(msigmas), float(msigmaa), int(tilex), int(tiley),
+                                          int(w), int(h), int(flags), C.byref(hnd)))
+        self._h = hnd
+        self._keep = (scene, keep, viewi, proji, depth)
+        self.scene, self.w, self.h = scene, int(w), int(h)
+        self.tilex, self.tiley = int(tilex), int(tiley)
+        self.scattevs, self.msigmas, self.msigmaa = int(scattevs), float(msigmas), float(msigmaa)
+        return True
+
+    def set_seed(self, seed, sample_base=0):
+        L.check(L.lib().drk_kernel_set_seed(self._h, int(seed), int(sample_base)))
+
+    def set_tuning(self, chunk=0, grid_blocks=0):
+        L.check(L.lib().drk_kernel_set_tuning(self._h, int(chunk), int(grid_blocks)))
+
+    def Render(self, x0, y0, tilex, tiley, samples):
+        """RenderKernel::Render (src/kernel.cpp:536-567); blocks until phb holds the sums."""
+        L.check(L.lib().drk_kernel_render(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples)))
+        self.tilew, self.tileh = int(tilex), int(tiley)
+
+    def Shadow(self, x0, y0, tilex, tiley, samples):
+        """RenderKernel::Shadow (src/kernel.cpp:569-607)."""
+        L.check(L.lib().drk_kernel_shadow(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples)))
+        self.tilew, self.tileh = int(tilex), int(tiley)
+
+    def _buf(self, b):
+        p = L.lib().drk_kernel_phb(self._h, b)
+        a = np.ctypeslib.as_array(p, shape=(self.tiley * self.tilex * 4,))
+        return a.reshape(self.tiley, self.tilex, 4)[: self.tileh, : self.tilew].copy()
+
+    @property
+    def phb(self):
+        """[lights, environment] per-pixel RGBA sums of the last Render (copies)."""
+        return [self._buf(0), self._buf(1)]
+
+    @property
+    def sumsq(self):
+        """Per-pixel sums of squares of the per-sample [lights, environment] RGB."""
+        return [self._buf(2), self._buf(3)]
+
+    def render_device(self, x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, sq_l=None, sq_e=None,
+                      stream=None):
+        """Device-buffer render into torch float32 tensors of tilex*tiley*4 floats (asynchronous)."""
+        def ptr(t):
+            if t is None:
+                return None
+            assert t.is_cuda and t.dtype.is_floating_point and t.is_contiguous() and t.numel() >= tilex * tiley * 4
+            return C.c_void_p(t.data_ptr())
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_kernel_render_device(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples),
+                                                 int(seed), int(sample_base), ptr(out_l), ptr(out_e), ptr(sq_l),
+                                                 ptr(sq_e), s))
+
+    def count(self, x0, y0, tilex, tiley, samples, seed, sample_base=0):
+        """Node-record reads and brick lookups of one render (counting pass)."""
+        c = (C.c_ulonglong * 2)()
+        L.check(L.lib().drk_kernel_count(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples), int(seed),
+                                         int(sample_base), c))
+        return int(c[0]), int(c[1])
+
+    def Destroy(self):
+        """RenderKernel::Destroy (src/kernel.cpp:609-615)."""
+        if self._h is not None:
+            L.lib().drk_kernel_destroy(self._h)
+            self._h = None
+            self._keep = None
+
+    def __del__(self):
+        try:
+            self.Destroy()
+        except Exception:
+            pass

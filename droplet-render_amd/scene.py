@@ -1,0 +1,94 @@
+"""Scene: the sparse octree and density bricks the render kernel reads.
+
+Mirrors droplet-render's `Scene` (src/scene.h:146-159): `ob` is the node array
+(OctreeStructure records), `pvol[VOLUME_BUFFER_FOG]` the bricks, `lvoxc` the
+brick edge, `leafx` the brick counts. `Scene.from_grid` replaces the OpenVDB
+part of Scene::Initialize (src/scene.cpp:840-917) for dense float grids.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Scene:
+    def __init__(self, ob, pvol_fog, lvoxc, mlevel=None, root_ce=None, voxel_size=None):
+        self.ob = np.ascontiguousarray(ob)            # structured node array (see node_dtype)
+        self.pvol = [None, np.ascontiguousarray(pvol_fog, dtype=np.float32)]
+        self.lvoxc = int(lvoxc)
+        self.lvoxc3 = self.lvoxc ** 3
+        self.leafx = [0, int(self.pvol[1].size // self.lvoxc3)]
+        self.index = len(self.ob) - 1
+        self.mlevel = mlevel
+        self.root_ce = root_ce
+        self.voxel_size = voxel_size
+        self._dev = None
+
+    node_dtype = np.dtype([("ce", "<f4", (4,)), ("chn", "<u4", (8,)), ("volx", "<u4", (2,)), ("qval", "<f4", (2,))])
+
+    @classmethod
+    def from_grid(cls, fog, voxel_size=None, maxd=12):
+        """Builds the octree from a dense (nz, ny, nx) float grid (x fastest);
+        voxel (i,j,k) sits at world (i,j,k)*voxel_size (default 1/nx)."""
+        fog = np.ascontiguousarray(fog, dtype=np.float32)
+        nz, ny, nx = fog.shape
+        s = float(voxel_size) if voxel_size else 1.0 / nx
+        out = C.POINTER(L.HostScene)()
+        L.check(L.lib().drk_scene_build_dense(L.fptr(fog), None, nx, ny, nz, s, maxd, C.byref(out)))
+        try:
+            hs = out.contents
+            n = hs.node_count
+            raw = np.ctypeslib.as_array(C.cast(hs.ob, C.POINTER(C.c_uint8)), shape=(n * 64,)).copy()
+            ob = raw.view(cls.node_dtype)
+            l3 = hs.lvoxc ** 3
+            nf = hs.leafx[1] * l3
+            pf = (np.ctypeslib.as_array(hs.pvol[1], shape=(nf,)).copy() if nf else np.zeros(0, np.float32))
+            sc = cls(ob, pf, hs.lvoxc, hs.mlevel, np.array(hs.root_ce[:], np.float32), s)
+        finally:
+            L.lib().drk_host_scene_free(out)
+        return sc
+
+    @property
+    def node_count(self):
+        return len(self.ob)
+
+    def occupancy(self):
+        """Fraction of leaves whose brick holds any density (leaf count over
+        the full 8^(mlevel-1) leaf lattice)."""
+        occ = int(np.count_nonzero(self.pvol[1].reshape(-1, self.lvoxc3).max(axis=1) > 0)) if self.leafx[1] else 0
+        return occ / float(8 ** (self.mlevel - 1)) if self.mlevel else 0.0
+
+    def device(self):
+        """Uploads the scene once and returns the opaque drk_scene handle."""
+        if self._dev is None:
+            h = C.c_void_p()
+            obp = self.ob.ctypes.data_as(C.POINTER(L.OctreeNode))
+            L.check(L.lib().drk_scene_create(obp, self.node_count, None, 0, L.fptr(self.pvol[1]), self.leafx[1],
+                                             self.lvoxc, C.byref(h)))
+            self._dev = h
+        return self._dev
+
+    def update_fog(self, pvol_fog, qval_fog=None):
+        """Replaces the FOG bricks (host array or device pointer via torch) and optionally the majorants."""
+        q = None if qval_fog is None else np.ascontiguousarray(qval_fog, dtype=np.float32)
+        if hasattr(pvol_fog, "data_ptr"):
+            ptr = C.cast(C.c_void_p(pvol_fog.data_ptr()), L._F)
+        else:
+            pvol_fog = np.ascontiguousarray(pvol_fog, dtype=np.float32)
+            self.pvol[1] = pvol_fog
+            ptr = L.fptr(pvol_fog)
+        L.check(L.lib().drk_scene_update_fog(self.device(), ptr, L.fptr(q)))
+        if q is not None:
+            self.ob["qval"][:, L.DRK_VOLUME_FOG] = q
+
+    def destroy(self):
+        if self._dev is not None:
+            L.lib().drk_scene_destroy(self._dev)
+            self._dev = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,205 @@
+/*
+ * droplet_render.h - C ABI of the MI355X volumetric light-transport path.
+ *
+ * This is the drop-in boundary under droplet-render's C++ render entry points.
+ * Each entry point names the reference interface it replaces (file:line in
+ * jaelpark/droplet-render). Plain pointers and sizes only; no exceptions
+ * cross this boundary. Every function returns 0 (DRK_OK) on success or a
+ * negative DRK_E* code, with a message available from drk_last_error().
+ *
+ * Threading: calls on one drk_kernel are serialised by the caller (the
+ * reference renders one tile at a time per process, src/main.cpp:650-667).
+ * Each drk_kernel owns its own HIP stream and device buffers.
+ */
+#ifndef DROPLET_RENDER_H
+#define DROPLET_RENDER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRK_OK        0
+#define DRK_EINVAL   -1   /* invalid argument (null pointer, empty/oversized tile, bad scene) */
+#define DRK_ENOMEM   -2   /* host or device allocation failed */
+#define DRK_EDEVICE  -3   /* no HIP device, launch or runtime failure */
+#define DRK_ELIMIT   -4   /* a bounded loop hit its iteration cap */
+
+/* Volume buffer slots, src/scene.h:4-8 (VOLUME_BUFFER_SDF, VOLUME_BUFFER_FOG). */
+#define DRK_VOLUME_SDF 0
+#define DRK_VOLUME_FOG 1
+#define DRK_NO_BRICK   0xffffffffu
+
+/* Octree node record, bit-identical to OctreeStructure (src/scene.h:21-31):
+ * 64 bytes, ce@0 chn@16 volx@48 qval@56. chn[] is indexed by the build slot
+ * i = x + 2y + 4z (src/scene.cpp:284-285); 0 means "no child". */
+typedef struct drk_octree_node {
+    float    ce[4];      /* centre xyz, half-extent in w */
+    uint32_t chn[8];
+    uint32_t volx[2];    /* brick index per volume buffer, DRK_NO_BRICK if none */
+    float    qval[2];    /* SDF: min distance; FOG: max density (majorant, >= 0.01) */
+} drk_octree_node;
+
+/* Host scene: the data Scene::Initialize leaves behind (src/scene.h:146-159). */
+typedef struct drk_host_scene {
+    drk_octree_node *ob;       /* node_count records, root at 0 */
+    uint32_t node_count;
+    float   *pvol[2];          /* bricks: leafx[b] * lvoxc^3 floats, vertex-centred, x fastest */
+    uint32_t leafx[2];
+    uint32_t lvoxc;            /* samples per brick edge (8 unless the depth cap doubled it) */
+    uint32_t mlevel;           /* octree depth; leaves sit at level mlevel-1 */
+    float    root_ce[4];
+} drk_host_scene;
+
+/*
+ * Builds the octree and bricks from dense grids, replacing the OpenVDB part of
+ * Scene::Initialize: leaf-AABB collection and root sizing
+ * (src/scene.cpp:676-716), Octree::BuildPath (src/scene.cpp:267-305) and the
+ * brick resampling with per-leaf majorants (src/scene.cpp:840-917).
+ * Grids are x-fastest, voxel (i,j,k) at world (i,j,k)*voxel_size; a voxel is
+ * active when its value differs from the background 0. sdf may be NULL.
+ * maxd is the depth cap (src/scene.cpp:712-716).
+ */
+int drk_scene_build_dense(const float *fog, const float *sdf,
+                          uint32_t nx, uint32_t ny, uint32_t nz,
+                          float voxel_size, uint32_t maxd,
+                          drk_host_scene **out);
+void drk_host_scene_free(drk_host_scene *scene);
+
+/* Device-resident scene: uploads Scene::ob and Scene::pvol once
+ * (the reference reads them in place, src/kernel.cpp:96-445). */
+typedef struct drk_scene drk_scene;
+int  drk_scene_create(const drk_octree_node *ob, uint32_t node_count,
+                      const float *pvol_sdf, uint32_t leafx_sdf,
+                      const float *pvol_fog, uint32_t leafx_fog,
+                      uint32_t lvoxc, drk_scene **out);
+/* Replaces the FOG bricks and per-leaf majorants in place (same leaf count);
+ * used between optimisation steps. qval may be NULL to keep the majorants. */
+int  drk_scene_update_fog(drk_scene *scene, const float *pvol_fog, const float *qval_fog);
+void drk_scene_destroy(drk_scene *scene);
+
+/* KernelSampler objects (src/KernelSampler.h:6-97). */
+#define DRK_PHASE_MIE 0      /* MiePhase::gmie, src/KernelSampler.cpp:206-287 */
+#define DRK_PHASE_HG  1      /* HGPhase(g),     src/KernelSampler.cpp:165-204 */
+#define DRK_ENV_HOSEK 0      /* Hosek-Wilkie sky, src/kernel.cpp:329-360 (USE_ARHOSEK_SKYMODEL) */
+#define DRK_ENV_NULL  1      /* NullEnv, src/KernelSampler.cpp:355-367 */
+#define DRK_ENV_MAP   2      /* MapEnv (equirect RGBA texels), src/KernelSampler.cpp:369-425 */
+#define DRK_MAX_LIGHTS 8
+
+/* SunLight(direction, color*intensity, angle), src/KernelSampler.cpp:305-345. */
+typedef struct drk_sun {
+    float direction[3];
+    float color[3];
+    float angle;
+} drk_sun;
+
+typedef struct drk_sampler_desc {
+    uint32_t phase;            /* DRK_PHASE_* */
+    float    hg_g;             /* HGPhase::g1 */
+    uint32_t env;              /* DRK_ENV_* */
+    float    sky_turbidity;    /* Hosek state parameters (reference hard-codes 7.0 / 0.15) */
+    float    sky_albedo;
+    const float *env_map;      /* env_w*env_h RGBA floats, row-major, for DRK_ENV_MAP */
+    uint32_t env_w, env_h;
+    const drk_sun *lights;     /* BaseLight::lights; lights[0] is the MIS light */
+    uint32_t light_count;
+} drk_sampler_desc;
+
+/* RenderKernel flags, src/kernel.h:4. */
+#define DRK_KERNEL_DEPTHCOMP 0x1
+
+typedef struct drk_kernel drk_kernel;
+
+/*
+ * RenderKernel::Initialize (src/kernel.h:15, src/kernel.cpp:497-534).
+ * viewi/proji are the inverted view and projection matrices as 4 rows of 4
+ * floats, row-vector convention (src/main.cpp:112-115). pdepth (w*h floats,
+ * NULL allowed) is the compositing depth. tilex*tiley sizes the two result
+ * buffers phb[0] (lights) and phb[1] (environment).
+ */
+int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float proji[16],
+                      const drk_sampler_desc *samplers, const float *pdepth,
+                      uint32_t scattevs, float msigmas, float msigmaa,
+                      uint32_t tilex, uint32_t tiley, uint32_t w, uint32_t h,
+                      uint32_t flags, drk_kernel **out);
+
+/* Seeds the counter-based RNG. Every Render advances sample_base by its sample
+ * count, so successive calls draw fresh samples (the reference's per-packet
+ * RNG streams keep running between calls, src/kernel.cpp:23-84). */
+int drk_kernel_set_seed(drk_kernel *k, uint64_t seed, uint32_t sample_base);
+
+/*
+ * RenderKernel::Render (src/kernel.h:16, src/kernel.cpp:536-567): renders the
+ * tile (x0,y0,tilex,tiley) with `samples` paths per pixel and leaves per-pixel
+ * SUMS (not means) in phb[0] (sun, alpha) and phb[1] (sky/env, alpha), tile
+ * row-major with stride tilex. Synchronous, like the reference. Odd tile
+ * sizes drop the last column/row as in src/kernel.cpp:453.
+ */
+int drk_kernel_render(drk_kernel *k, uint32_t x0, uint32_t y0,
+                      uint32_t tilex, uint32_t tiley, uint32_t samples);
+
+/* RenderKernel::Shadow (src/kernel.h:17, src/kernel.cpp:569-607). */
+int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0,
+                      uint32_t tilex, uint32_t tiley, uint32_t samples);
+
+/* RenderKernel::phb[buffer] (src/kernel.h:26): tilex*tiley RGBA floats,
+ * valid until the next Render/Shadow or Destroy. buffer 2 and 3 hold the
+ * per-pixel sums of squares of the lights and environment samples
+ * (needed for Monte Carlo error bars; no reference counterpart). */
+const float *drk_kernel_phb(const drk_kernel *k, uint32_t buffer);
+
+/* RenderKernel::Destroy (src/kernel.h:18, src/kernel.cpp:609-615). */
+void drk_kernel_destroy(drk_kernel *k);
+
+/*
+ * Device-buffer form of Render for callers that keep results in HBM
+ * (benchmarks, the autograd wrapper, batched drivers). Output pointers are
+ * device pointers of tilex*tiley float4 each; the sums are written (not
+ * accumulated). sumsq pointers may be NULL. stream is a hipStream_t (NULL =
+ * the kernel's own stream). Asynchronous on that stream.
+ */
+int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
+                             uint32_t tilex, uint32_t tiley, uint32_t samples,
+                             uint64_t seed, uint32_t sample_base,
+                             float *d_lights, float *d_env,
+                             float *d_sumsq_lights, float *d_sumsq_env,
+                             void *stream);
+
+/*
+ * Extensions with no reference counterpart.
+ *
+ * drk_kernel_set_tuning: fixes the work-unit chunk (samples per unit) and the
+ * persistent grid size instead of the built-in heuristic; 0 keeps the
+ * heuristic. Used by the autotuner and benchmarks.
+ *
+ * drk_kernel_count: runs the render once with per-lane counters and returns
+ * counts[0] = octree node records read, counts[1] = brick lookups. The RNG is
+ * counter-based, so the counts describe any run with the same arguments.
+ */
+int drk_kernel_set_tuning(drk_kernel *k, uint32_t chunk, uint32_t grid_blocks);
+int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                     uint32_t samples, uint64_t seed, uint32_t sample_base,
+                     unsigned long long counts[2]);
+
+/*
+ * Synthetic procedural density grid (n^3, x fastest, values in [0,1]) used as
+ * benchmark and test input in place of the reference's Blender/OpenVDB scene
+ * construction. kind 0: fBm cloud, prm = {octaves, base frequency,
+ * lacunarity, gain, threshold}; kind 1: fBm-displaced union of spheres,
+ * prm = {spheres, centre min, centre max, radius min, radius max,
+ * displacement amplitude, octaves, base frequency, band width}.
+ */
+int drk_synth_cloud(uint32_t kind, uint32_t n, uint64_t seed, const float *prm, uint32_t nprm, float *out);
+
+/* Thread-local text of the last error. */
+const char *drk_last_error(void);
+
+/* Library/device information (for logs and benchmarks). */
+int drk_device_info(char *name, uint32_t name_len, uint32_t *cu_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,730 @@
+/*
+ * CPU oracle of droplet-render's volumetric light-transport path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this restatement is the checker the parity tests
+ * compare the gfx950 kernels against, and the CPU baseline that bench.py
+ * times beside them. The product path never links or calls it.
+ *
+ * Plain C, scalar, one lane per path, written from the reference's behaviour
+ * (jaelpark/droplet-render):
+ *   ray generation            src/kernel.cpp:447-487 (K_ParallelRender)
+ *   estimator (recursive)     src/kernel.cpp:128-445 (SampleVolume), followed
+ *                             structurally: the recursion, the masks per lane,
+ *                             the leaf loop and the Woodcock loop
+ *   trilinear lookup          src/kernel.cpp:96-126 (SampleVoxelSpace)
+ *   octree traversal          src/KernelOctree.cpp:9-220 (recursive full traverser)
+ *   Mie / HG / sun            src/KernelSampler.cpp:145-345
+ *   sky                       src/kernel.cpp:329-360, src/ArHosekSkyModel.cpp:147-339
+ *   octree build              src/scene.cpp:234-305, :676-762, :840-917
+ * Deliberate, shared deviations from the reference (also in the kernels):
+ *   - RNG: Philox-4x32-10 keyed by (pixel, seed) seeds one xoshiro128**
+ *     stream per (sample, ray); uniforms lie in (0,1) strictly;
+ *   - acos/cos/sin/log/exp/pow are the C library's, not the reference's
+ *     polynomial/SSE approximations;
+ *   - Hosek turbidity/albedo are parameters (the reference hard-codes 7/0.15).
+ * Compiled with -ffp-contract=off so every multiply-add rounds twice, as the
+ * reference's default SSE3 build does.
+ */
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ONO 0xffffffffu
+
+typedef struct {
+    float ce[4];
+    uint32_t chn[8];
+    uint32_t volx[2];
+    float qval[2];
+} onode;
+
+/* ------------------------------------------------------------------ RNG -- */
+static void philox(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void or_philox(const uint32_t *ctr, const uint32_t *key, uint32_t n, uint32_t *out) {
+    for (uint32_t i = 0; i < n; ++i) philox(ctr + 4 * i, key[2 * i], key[2 * i + 1], out + 4 * i);
+}
+
+static float u01(uint32_t x) { return (float)((x >> 8) | 1u) * 5.9604644775390625e-8f; }
+
+typedef struct { uint32_t s[4]; } ostream;
+
+static void stream_seed(ostream *r, uint32_t key0, uint32_t key1, uint32_t sample, uint32_t ray) {
+    uint32_t c[4] = {sample, ray, 0u, 0u}, v[4];
+    philox(c, key0, key1, v);
+    r->s[0] = v[0]; r->s[1] = v[1]; r->s[2] = v[2];
+    r->s[3] = v[3] | ((v[0] | v[1] | v[2]) == 0u ? 1u : 0u);
+}
+
+static uint32_t stream_next(ostream *r) {
+    uint32_t *s = r->s;
+    uint32_t x = s[1] * 5u;
+    uint32_t res = ((x << 7) | (x >> 25)) * 9u;
+    uint32_t t = s[1] << 9;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t;
+    s[3] = (s[3] << 11) | (s[3] >> 21);
+    return res;
+}
+
+static float stream_u(ostream *r) { return u01(stream_next(r)); }
+
+void or_ray_stream(const uint32_t *key0, const uint32_t *sample, const uint32_t *ray, uint32_t n, uint32_t key1,
+                   uint32_t m, float *out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        ostream r;
+        stream_seed(&r, key0[i], key1, sample[i], ray[i]);
+        for (uint32_t j = 0; j < m; ++j) out[(size_t)i * m + j] = stream_u(&r);
+    }
+}
+
+/* ------------------------------------------------------------- octree -- */
+typedef struct { uint32_t node; float t0, t1; } oleaf;
+
+typedef struct {
+    const onode *ob;
+    oleaf *ls;
+    uint32_t n, cap;
+} olist;
+
+static void push_leaf(olist *L, uint32_t node, float a, float b) {
+    if (L->n == L->cap) {
+        L->cap = L->cap ? 2 * L->cap : 64;
+        L->ls = (oleaf *)realloc(L->ls, sizeof(oleaf) * L->cap);
+    }
+    L->ls[L->n].node = node; L->ls[L->n].t0 = a; L->ls[L->n].t1 = b;
+    L->n++;
+}
+
+static uint32_t first_node(const float t0[3], const float tm[3]) {
+    uint32_t a = 0;
+    if (t0[0] > t0[1]) {
+        if (t0[0] > t0[2]) {
+            if (tm[1] < t0[0]) a |= 2;
+            if (tm[2] < t0[0]) a |= 1;
+            return a;
+        }
+    } else {
+        if (t0[1] > t0[2]) {
+            if (tm[0] < t0[1]) a |= 4;
+            if (tm[2] < t0[1]) a |= 1;
+            return a;
+        }
+    }
+    if (tm[0] < t0[2]) a |= 4;
+    if (tm[1] < t0[2]) a |= 2;
+    return a;
+}
+
+static uint32_t next_node(const float tm[3], uint32_t x, uint32_t y, uint32_t z) {
+    if (tm[0] < tm[1]) {
+        if (tm[0] < tm[2]) return x;
+    } else {
+        if (tm[1] < tm[2]) return y;
+    }
+    return z;
+}
+
+static float fmax3(const float v[3]) { return fmaxf(fmaxf(v[0], v[1]), v[2]); }
+static float fmin3(const float v[3]) { return fminf(fminf(v[0], v[1]), v[2]); }
+static int is_leaf(const onode *n) { return n->volx[0] != ONO || n->volx[1] != ONO; }
+
+/* OctreeFullTraverser::OctreeProcessSubtree (KernelOctree.cpp:143-200). */
+static void process_subtree(olist *L, const float t0[3], const float t1[3], uint32_t a, uint32_t n, uint32_t l) {
+    static const uint32_t nla[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+    if (t1[0] < 0.0f || t1[1] < 0.0f || t1[2] < 0.0f || (n == 0 && l > 0)) return;
+    const onode *N = &L->ob[n];
+    if (is_leaf(N)) {
+        push_leaf(L, n, fmax3(t0), fmin3(t1));
+        return;
+    }
+    float tm[3] = {0.5f * (t0[0] + t1[0]), 0.5f * (t0[1] + t1[1]), 0.5f * (t0[2] + t1[2])};
+    uint32_t tn = first_node(t0, tm);
+    do {
+        float a0[3], a1[3];
+        switch (tn) {
+        case 0:
+            process_subtree(L, t0, tm, a, N->chn[nla[a]], l + 1);
+            tn = next_node(tm, 4, 2, 1);
+            break;
+        case 1:
+            a0[0] = t0[0]; a0[1] = t0[1]; a0[2] = tm[2]; a1[0] = tm[0]; a1[1] = tm[1]; a1[2] = t1[2];
+            process_subtree(L, a0, a1, a, N->chn[nla[1 ^ a]], l + 1);
+            tn = next_node(a1, 5, 3, 8);
+            break;
+        case 2:
+            a0[0] = t0[0]; a0[1] = tm[1]; a0[2] = t0[2]; a1[0] = tm[0]; a1[1] = t1[1]; a1[2] = tm[2];
+            process_subtree(L, a0, a1, a, N->chn[nla[2 ^ a]], l + 1);
+            tn = next_node(a1, 6, 8, 3);
+            break;
+        case 3:
+            a0[0] = t0[0]; a0[1] = tm[1]; a0[2] = tm[2]; a1[0] = tm[0]; a1[1] = t1[1]; a1[2] = t1[2];
+            process_subtree(L, a0, a1, a, N->chn[nla[3 ^ a]], l + 1);
+            tn = next_node(a1, 7, 8, 8);
+            break;
+        case 4:
+            a0[0] = tm[0]; a0[1] = t0[1]; a0[2] = t0[2]; a1[0] = t1[0]; a1[1] = tm[1]; a1[2] = tm[2];
+            process_subtree(L, a0, a1, a, N->chn[nla[4 ^ a]], l + 1);
+            tn = next_node(a1, 8, 6, 5);
+            break;
+        case 5:
+            a0[0] = tm[0]; a0[1] = t0[1]; a0[2] = tm[2]; a1[0] = t1[0]; a1[1] = tm[1]; a1[2] = t1[2];
+            process_subtree(L, a0, a1, a, N->chn[nla[5 ^ a]], l + 1);
+            tn = next_node(a1, 8, 7, 8);
+            break;
+        case 6:
+            a0[0] = tm[0]; a0[1] = tm[1]; a0[2] = t0[2]; a1[0] = t1[0]; a1[1] = t1[1]; a1[2] = tm[2];
+            process_subtree(L, a0, a1, a, N->chn[nla[6 ^ a]], l + 1);
+            tn = next_node(a1, 8, 8, 7);
+            break;
+        case 7:
+            process_subtree(L, tm, t1, a, N->chn[nla[7 ^ a]], l + 1);
+            tn = 8;
+            break;
+        }
+    } while (tn < 8);
+}
+
+/* OctreeInitialize (KernelOctree.cpp:78-110) + the root test of :137. */
+static void traverse(olist *L, const float ro[3], const float rd[3]) {
+    L->n = 0;
+    const float *ce = L->ob[0].ce;
+    float o[3], d[3];
+    uint32_t a = 0;
+    for (int k = 0; k < 3; ++k) {
+        o[k] = (ro[k] - ce[k]) + ce[3];
+        d[k] = rd[k];
+    }
+    if (d[0] < 0.0f) { o[0] = 2.0f * ce[3] - o[0]; d[0] = -d[0]; a |= 4; }
+    if (d[1] < 0.0f) { o[1] = 2.0f * ce[3] - o[1]; d[1] = -d[1]; a |= 2; }
+    if (d[2] < 0.0f) { o[2] = 2.0f * ce[3] - o[2]; d[2] = -d[2]; a |= 1; }
+    float t0[3], t1[3], e2 = 2.0f * ce[3];
+    for (int k = 0; k < 3; ++k) {
+        t0[k] = (0.0f - o[k]) / d[k];
+        t1[k] = (e2 - o[k]) / d[k];
+    }
+    if (fmax3(t0) < fmin3(t1)) process_subtree(L, t0, t1, a, 0, 0);
+}
+
+uint32_t or_traverse(const onode *ob, const float *ro, const float *rd, uint32_t n, uint32_t max_leaves,
+                     uint32_t *nodes, float *tr0, float *tr1, uint32_t *count) {
+    olist L = {ob, NULL, 0, 0};
+    for (uint32_t i = 0; i < n; ++i) {
+        traverse(&L, ro + 3 * i, rd + 3 * i);
+        count[i] = L.n;
+        for (uint32_t j = 0; j < L.n && j < max_leaves; ++j) {
+            nodes[(size_t)i * max_leaves + j] = L.ls[j].node;
+            tr0[(size_t)i * max_leaves + j] = L.ls[j].t0;
+            tr1[(size_t)i * max_leaves + j] = L.ls[j].t1;
+        }
+    }
+    free(L.ls);
+    return 0;
+}
+
+/* ------------------------------------------------------- brick lookup -- */
+/* SampleVoxelSpace (kernel.cpp:96-126). */
+float or_voxel(const float *pvol, const float p[3], const float ce[4], uint32_t lvoxc) {
+    float nv = (float)lvoxc, nvm = nv - 1.0f, ni[3], nl[3];
+    uint32_t a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        ni[k] = ((-0.5f * ((ce[k] - ce[3]) - p[k])) * nvm) / ce[3];
+        float nf = fmaxf(floorf(ni[k]), 0.0f);
+        float nc = fminf(ceilf(ni[k]), nvm);
+        nl[k] = ni[k] - nf;
+        nc = fmaxf(fminf(nc, (float)(lvoxc - 1)), 0.0f);
+        nf = fmaxf(fminf(nf, (float)(lvoxc - 1)), 0.0f);
+        a[k] = (uint32_t)nf;
+        b[k] = (uint32_t)nc;
+    }
+    uint32_t l2 = lvoxc * lvoxc;
+    float ua[4] = {pvol[a[2] * l2 + a[1] * lvoxc + a[0]], pvol[a[2] * l2 + b[1] * lvoxc + a[0]],
+                   pvol[b[2] * l2 + a[1] * lvoxc + a[0]], pvol[b[2] * l2 + b[1] * lvoxc + a[0]]};
+    float ub[4] = {pvol[a[2] * l2 + a[1] * lvoxc + b[0]], pvol[a[2] * l2 + b[1] * lvoxc + b[0]],
+                   pvol[b[2] * l2 + a[1] * lvoxc + b[0]], pvol[b[2] * l2 + b[1] * lvoxc + b[0]]};
+    float u[4];
+    for (int k = 0; k < 4; ++k) u[k] = nl[0] * (ub[k] - ua[k]) + ua[k];
+    float v0 = nl[1] * (u[1] - u[0]) + u[0];
+    float v1 = nl[1] * (u[3] - u[2]) + u[2];
+    return nl[2] * (v1 - v0) + v0;
+}
+
+/* ------------------------------------------------------------ samplers -- */
+typedef struct {
+    float pdf[256][3];
+    float cdf[256][3];
+} omie;
+
+static void basis(const float iv[3], float b1[3], float b2[3]) {
+    float len = sqrtf(iv[0] * iv[0] + 0.0f * 0.0f + iv[2] * iv[2]);
+    b1[0] = -iv[2] / len; b1[1] = 0.0f / len; b1[2] = iv[0] / len;
+    b2[0] = iv[1] * b1[2] - iv[2] * b1[1];
+    b2[1] = iv[2] * b1[0] - iv[0] * b1[2];
+    b2[2] = iv[0] * b1[1] - iv[1] * b1[0];
+}
+
+static void basis_dir(const float iv[3], float st, float ct, float ph, float out[3]) {
+    float b1[3], b2[3];
+    basis(iv, b1, b2);
+    float sph = sinf(ph), cph = cosf(ph);
+    for (int k = 0; k < 3; ++k) out[k] = b1[k] * st * cph + b2[k] * st * sph + iv[k] * ct;
+}
+
+/* MiePhase::EvaluateRGB (KernelSampler.cpp:233-256). */
+void or_mie_eval(const omie *M, float ct, float out[3]) {
+    float ct1 = fmaxf(fminf(ct, 1.0f), -1.0f);
+    float a = acosf(-ct1) / 3.14159265358979f;
+    float b = 256.0f * a;
+    float c = floorf(b);
+    c = fminf(c, 254.0f);
+    c = fmaxf(c, 0.0f);
+    float t = b - c;
+    uint32_t i = (uint32_t)c;
+    for (int k = 0; k < 3; ++k) out[k] = t * (M->pdf[i + 1][k] - M->pdf[i][k]) + M->pdf[i][k];
+}
+
+/* MiePhase::Sample's quantised inverse CDF (KernelSampler.cpp:258-273): the
+ * linear scan over i = 0..254, keeping s for every i whose key is below u1
+ * and stopping at the first that is not. Returns th/pi. */
+float or_mie_theta(const omie *M, float u1) {
+    float th = 0.0f;
+    float ub = 256.0f * u1;
+    for (uint32_t i = 0; i < 255; ++i) {
+        float c0 = M->cdf[i][0], c1 = M->cdf[i + 1][0];
+        float b = fmaxf(ub, (float)i);
+        float c = fminf(floorf(b), 255.0f);
+        float t = b - c;
+        float cdf = t * (c1 - c0) + c0;
+        if (!(cdf < u1)) break;
+        th = (float)i / 256.0f + t / 256.0f;
+    }
+    return th;
+}
+
+void or_mie_sample(const omie *M, const float iv[3], float u1, float u2, float out[3]) {
+    float th = or_mie_theta(M, u1) * 3.14159265358979f;
+    float ph = (2.0f * 3.14159265358979f) * u2;
+    float ct = -cosf(th);
+    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+    basis_dir(iv, st, ct, ph, out);
+}
+
+float or_hg_eval(float g, float ct) {
+    return (1.0f - g * g) / (4.0f * 3.14159265358979f * powf(1.0f + g * g - 2.0f * g * ct, 1.5f));
+}
+
+void or_hg_sample(float g, const float iv[3], float u1, float u2, float out[3]) {
+    float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+    float ct = (1.0f + g * g - sq * sq) / (2.0f * g);
+    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+    float ph = (2.0f * 3.14159265358979f) * u2;
+    basis_dir(iv, st, ct, ph, out);
+}
+
+typedef struct {
+    float dir[3], color[3], cosang;
+} olight;
+
+static float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* SunLight::Pdf (KernelSampler.cpp:326-330). */
+static float sun_pdf(const olight *l, const float v[3]) {
+    float ctm = l->cosang;
+    return dot3(v, l->dir) > ctm ? 1.0f / (2.0f * 3.14159265358979f * (1.0f - ctm * ctm)) : 0.0f;
+}
+
+/* SunLight::Sample (KernelSampler.cpp:332-345). */
+static void sun_sample(const olight *l, float u1, float u2, float out[3]) {
+    float ctm = l->cosang;
+    float ct = (1.0f - u1) + u1 * ctm;
+    float st = sqrtf(1.0f - ct * ct);
+    float ph = (2.0f * 3.14159265358979f) * u2;
+    basis_dir(l->dir, st, ct, ph, out);
+}
+
+void or_sun(const float dir[3], const float color[3], float angle, const float *u1, const float *u2,
+            const float *dirs, uint32_t n, float *osd, float *opdf, float *oev) {
+    olight l;
+    memcpy(l.dir, dir, 12);
+    memcpy(l.color, color, 12);
+    l.cosang = cosf(angle);
+    for (uint32_t i = 0; i < n; ++i) {
+        sun_sample(&l, u1[i], u2[i], osd + 3 * i);
+        opdf[i] = sun_pdf(&l, dirs + 3 * i);
+        int in = dot3(dirs + 3 * i, l.dir) > l.cosang;
+        for (int k = 0; k < 3; ++k) oev[3 * i + k] = in ? l.color[k] : 0.0f;
+    }
+}
+
+/* --------------------------------------------------------------- sky --- */
+/* Hosek-Wilkie state (ArHosekSkyModel.cpp:147-289), double. */
+static void hosek_cook(const double *dataset, double *config, double turbidity, double albedo, double elev) {
+    int it = (int)turbidity;
+    double tr = turbidity - (double)it;
+    double x = pow(elev / (3.141592653589793 / 2.0), 1.0 / 3.0);
+    double wa[4] = {(1.0 - albedo) * (1.0 - tr), albedo * (1.0 - tr), (1.0 - albedo) * tr, albedo * tr};
+    int ai[4] = {0, 1, 0, 1}, ti[4] = {it - 1, it - 1, it, it};
+    for (int i = 0; i < 9; ++i) config[i] = 0.0;
+    for (int k = 0; k < (it == 10 ? 2 : 4); ++k) {
+        const double *m = dataset + 9 * 6 * 10 * ai[k] + 9 * 6 * ti[k];
+        for (int i = 0; i < 9; ++i)
+            config[i] += wa[k] * (pow(1.0 - x, 5.0) * m[i] + 5.0 * pow(1.0 - x, 4.0) * x * m[i + 9] +
+                                  10.0 * pow(1.0 - x, 3.0) * pow(x, 2.0) * m[i + 18] +
+                                  10.0 * pow(1.0 - x, 2.0) * pow(x, 3.0) * m[i + 27] +
+                                  5.0 * (1.0 - x) * pow(x, 4.0) * m[i + 36] + pow(x, 5.0) * m[i + 45]);
+    }
+}
+
+static double hosek_cook_rad(const double *dataset, double turbidity, double albedo, double elev) {
+    int it = (int)turbidity;
+    double tr = turbidity - (double)it;
+    double x = pow(elev / (3.141592653589793 / 2.0), 1.0 / 3.0);
+    double wa[4] = {(1.0 - albedo) * (1.0 - tr), albedo * (1.0 - tr), (1.0 - albedo) * tr, albedo * tr};
+    int ai[4] = {0, 1, 0, 1}, ti[4] = {it - 1, it - 1, it, it};
+    double res = 0.0;
+    for (int k = 0; k < (it == 10 ? 2 : 4); ++k) {
+        const double *m = dataset + 6 * 10 * ai[k] + 6 * ti[k];
+        res += wa[k] * (pow(1.0 - x, 5.0) * m[0] + 5.0 * pow(1.0 - x, 4.0) * x * m[1] +
+                        10.0 * pow(1.0 - x, 3.0) * pow(x, 2.0) * m[2] + 10.0 * pow(1.0 - x, 2.0) * pow(x, 3.0) * m[3] +
+                        5.0 * (1.0 - x) * pow(x, 4.0) * m[4] + pow(x, 5.0) * m[5]);
+    }
+    return res;
+}
+
+/* coeffs: 3 x 1080 doubles, rad: 3 x 120 doubles (the RGB dataset). */
+void or_hosek_state(const double *coeffs, const double *rad, double turbidity, double albedo, double elev,
+                    double *cfg27, double *rad3) {
+    for (int ch = 0; ch < 3; ++ch) {
+        hosek_cook(coeffs + 1080 * ch, cfg27 + 9 * ch, turbidity, albedo, elev);
+        rad3[ch] = hosek_cook_rad(rad + 120 * ch, turbidity, albedo, elev);
+    }
+}
+
+typedef struct {
+    float cfg[3][9], rad[3], dir[3];
+} osky;
+
+/* src/kernel.cpp:329-360 for one direction, float. */
+static void sky_eval(const osky *S, const float rd[3], float out[3]) {
+    float rdz = rd[2];
+    float sth = sqrtf(1.0f - rd[2] * rd[2]);
+    float cth = rdz;
+    float slth = sqrtf(1.0f - S->dir[2] * S->dir[2]);
+    float clth = S->dir[2], slph = S->dir[1], clph = S->dir[0];
+    float rdd = rd[0] / rd[1];
+    float acr = 1.0f / sqrtf(rdd * rdd + 1.0f);
+    float czp = acr * (rdd * slph + clph);
+    float cph = sth * slth * czp + cth * clth;
+    float gmma = acosf(cph);
+    float gacs = fmaxf(fminf(cph, 1.0f), 0.0f);
+    float thcs = fmaxf(rdz, 0.0f);
+    float raym = gacs * gacs;
+    for (int i = 0; i < 3; ++i) {
+        const float *c = S->cfg[i];
+        float expm = expf(c[4] * gmma);
+        float miem = (1.0f + raym) / powf(1.0f + c[8] * c[8] - 2.0f * c[8] * gacs, 1.5f);
+        float zenh = sqrtf(thcs);
+        float ca = (1.0f + c[0] * expf(c[1] / (thcs + 0.01f))) * (c[2] + c[3] * expm + c[5] * raym + c[6] * miem + c[7] * zenh);
+        ca *= S->rad[i];
+        out[i] = 1e-3f * powf(ca, 2.2f);
+    }
+}
+
+void or_sky(const float cfg27[27], const float rad3[3], const float dir[3], const float *rd, uint32_t n, float *out) {
+    osky S;
+    memcpy(S.cfg, cfg27, sizeof(S.cfg));
+    memcpy(S.rad, rad3, 12);
+    memcpy(S.dir, dir, 12);
+    for (uint32_t i = 0; i < n; ++i) sky_eval(&S, rd + 3 * i, out + 3 * i);
+}
+
+/* MapEnv::Evaluate (KernelSampler.cpp:369-421) then pow 2.2. */
+static void map_env(const float *tex, uint32_t w, uint32_t h, const float rd[3], float out[3]) {
+    float tha = acosf(-rd[2]) / 3.14159265358979f;
+    float thb = (float)h * fmaxf(fminf(tha, 1.0f), 0.0f);
+    float thc = fmaxf(fminf(floorf(thb), (float)(h - 2)), 0.0f);
+    float tht = thb - thc;
+    float pha = 0.5f + atan2f(rd[1], -rd[0]) / (2.0f * 3.14159265358979f);
+    float phb = (float)w * fmaxf(fminf(pha, 1.0f), 0.0f);
+    float phc = fmaxf(fminf(floorf(phb), (float)(w - 2)), 0.0f);
+    float pht = phb - phc;
+    uint32_t ti = (uint32_t)thc, pj = (uint32_t)phc;
+    for (int i = 0; i < 3; ++i) {
+        float ca = tex[(ti * w + pj) * 4 + i], cb = tex[((ti + 1) * w + pj) * 4 + i];
+        float da = tex[(ti * w + pj + 1) * 4 + i], db = tex[((ti + 1) * w + pj + 1) * 4 + i];
+        float l0 = tht * (cb - ca) + ca, l1 = tht * (db - da) + da;
+        out[i] = powf(pht * (l1 - l0) + l0, 2.2f);
+    }
+}
+
+/* ----------------------------------------------------------- renderer -- */
+typedef struct {
+    const onode *ob;
+    const float *pvol_fog;
+    uint32_t lvoxc, lvoxc3;
+    float viewi[16], proji[16];
+    uint32_t w, h;
+    uint32_t scattevs;
+    float sigma_s, sigma_a;
+    uint32_t phase;          /* 0 Mie, 1 HG */
+    float hg_g;
+    uint32_t env;            /* 0 Hosek, 1 null, 2 map */
+    const float *env_map;
+    uint32_t env_w, env_h;
+    uint32_t light_count;
+    olight lights[8];
+    osky sky;
+    const omie *mie;
+    const float *depth;      /* w*h or NULL */
+    uint32_t key1;
+} oparams;
+
+typedef struct {
+    const oparams *P;
+    olist L;
+    uint32_t key0, sample;
+} octx;
+
+/* Result of one SampleVolume call for one lane: cl/cs RGBA sums. */
+typedef struct { float cl[4], cs[4]; } ores;
+
+static void env_eval(const oparams *P, const float rd[3], float le[3]) {
+    if (P->env == 0) sky_eval(&P->sky, rd, le);
+    else if (P->env == 2) map_env(P->env_map, P->env_w, P->env_h, rd, le);
+    else le[0] = le[1] = le[2] = 0.0f;
+}
+
+/* Leaf loop + Woodcock loop of SampleVolume (kernel.cpp:197-317) for one lane
+ * with FOG-only leaves. Returns 1 if the ray collided (rm == false), with the
+ * distance in *ptd. */
+static int track(octx *X, const float ro[3], const float rd[3], uint32_t ray_id, float depth, float *ptd) {
+    const oparams *P = X->P;
+    ostream rng;
+    stream_seed(&rng, X->key0, P->key1, X->sample, ray_id);
+    traverse(&X->L, ro, rd);
+    const float sige = P->sigma_a + P->sigma_s;
+    float td = 0.0f;
+    int rm = 1;
+    for (uint32_t i = 0;; ++i) {
+        if (i >= X->L.n) break;          /* lm == 0 */
+        if (!(td < depth)) break;        /* mm == 0 */
+        const oleaf *lf = &X->L.ls[i];
+        const onode *N = &P->ob[lf->node];
+        float tra = lf->t0, trb = lf->t1;
+        float lo = td;
+        float tr0 = tra - td, tr1 = trb - td;
+        float smax = N->qval[1];
+        /* sole fog leaf: sm = true, skip to the leaf entry */
+        td = tra;
+        float s0 = tr0;
+        const float *brick = P->pvol_fog + (size_t)N->volx[1] * P->lvoxc3;
+        float sr = -(logf(stream_u(&rng)) / (sige * smax));
+        int sm = 1;
+        for (;;) {
+            if (!(sm && rm)) break;
+            float sc = s0 + sr;
+            td = lo + sc;
+            sm = sc < tr1;
+            if (!sm) td = trb;
+            if (sm) rm = sc > tr0;
+            if (sm && rm) {
+                float rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
+                float p = or_voxel(brick, rc, N->ce, P->lvoxc);
+                float q = stream_u(&rng);
+                rm = p / smax < q;
+                if (rm) sr = sr - logf(stream_u(&rng)) / (sige * smax);
+            }
+        }
+        if (!rm) break;
+    }
+    *ptd = td;
+    return !rm;
+}
+
+/* SampleVolume (kernel.cpp:128-445) for one lane, samples = 1, depth r,
+ * tracking ray `ray_id`. */
+static void sample_volume(octx *X, const float ro[3], const float rd[3], uint32_t r, uint32_t ray_id, float depth,
+                          ores *out) {
+    const oparams *P = X->P;
+    memset(out, 0, sizeof(*out));
+    float td;
+    int scattered = track(X, ro, rd, ray_id, depth, &td);
+    int rm = !scattered;
+    float lc[4] = {0, 0, 0, 0}, le[4] = {0, 0, 0, 0};
+    if (rm && r > 0) {
+        for (uint32_t l = 0; l < P->light_count; ++l)
+            if (dot3(rd, P->lights[l].dir) > P->lights[l].cosang)
+                for (int k = 0; k < 3; ++k) lc[k] += P->lights[l].color[k];
+        env_eval(P, rd, le);
+        lc[3] = le[3] = 1.0f;
+    } else {
+        lc[3] = le[3] = rm ? 0.0f : 1.0f;
+    }
+    if (r < P->scattevs && !rm) {
+        uint32_t c[4] = {X->sample, r, 0u, 1u}, v[4];
+        philox(c, X->key0, P->key1, v);
+        float u1 = u01(v[0]), u2 = u01(v[1]), u3 = u01(v[2]), u4 = u01(v[3]);
+        float srd[3], lrd[3], p1[3], p3[3];
+        if (P->phase == 0) or_mie_sample(P->mie, rd, u1, u2, srd);
+        else or_hg_sample(P->hg_g, rd, u1, u2, srd);
+        sun_sample(&P->lights[0], u3, u4, lrd);
+        if (P->phase == 0) {
+            or_mie_eval(P->mie, dot3(srd, rd), p1);
+            or_mie_eval(P->mie, dot3(lrd, rd), p3);
+        } else {
+            float h1 = or_hg_eval(P->hg_g, dot3(srd, rd)), h3 = or_hg_eval(P->hg_g, dot3(lrd, rd));
+            p1[0] = p1[1] = p1[2] = h1;
+            p3[0] = p3[1] = p3[2] = h3;
+        }
+        float p2 = sun_pdf(&P->lights[0], lrd);
+        float rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
+        ores S1, S2;
+        sample_volume(X, rc, srd, r + 1, 2u * r + 2u, FLT_MAX, &S1);
+        sample_volume(X, rc, lrd, P->scattevs, 2u * r + 1u, FLT_MAX, &S2);
+        float pd1 = sun_pdf(&P->lights[0], srd);
+        float sige = P->sigma_a + P->sigma_s;
+        for (int k = 0; k < 3; ++k) {
+            float w1 = p1[k] / (p1[k] + pd1);
+            float w2 = p3[k] / (p3[k] + p2);
+            float cl1 = ((S1.cl[k] * w1 + S2.cl[k] * w2) * P->sigma_s) / sige;
+            float cs1 = ((S1.cs[k] * w1 + 0.0f * w2) * P->sigma_s) / sige;
+            out->cl[k] += cl1;
+            out->cs[k] += cs1;
+        }
+        out->cl[3] += lc[3];
+        out->cs[3] += le[3];
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            out->cl[k] += rm ? lc[k] : 0.0f;
+            out->cs[k] += rm ? le[k] : 0.0f;
+        }
+        out->cl[3] += lc[3];
+        out->cs[3] += le[3];
+    }
+}
+
+/* Primary ray of pixel (px,py), K_ParallelRender (kernel.cpp:461-474). */
+static int camera_ray(const oparams *P, uint32_t px, uint32_t py, float ro[3], float rd[3]) {
+    float hx = -(2.0f * ((float)px + 0.5f) / (float)P->w - 1.0f);
+    float hy = -(2.0f * ((float)py + 0.5f) / (float)P->h - 1.0f);
+    const float *pi = P->proji, *vi = P->viewi;
+    float rv[3], d[3];
+    for (int j = 0; j < 3; ++j) rv[j] = hx * pi[j] + hy * pi[4 + j] + 0.0f * pi[8 + j] + 1.0f * pi[12 + j];
+    for (int j = 0; j < 3; ++j) d[j] = rv[0] * vi[j] + rv[1] * vi[4 + j] + rv[2] * vi[8 + j] + 0.0f * vi[12 + j];
+    float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int j = 0; j < 3; ++j) {
+        rd[j] = d[j] / len;
+        ro[j] = vi[12 + j];
+    }
+    return hx > -1.0f && hx < 1.0f && hy > -1.0f && hy < 1.0f;
+}
+
+void or_camera(const float viewi[16], const float proji[16], uint32_t w, uint32_t h, const uint32_t *px,
+               const uint32_t *py, uint32_t n, float *ro, float *rd, uint32_t *gm) {
+    oparams P;
+    memset(&P, 0, sizeof(P));
+    memcpy(P.viewi, viewi, 64);
+    memcpy(P.proji, proji, 64);
+    P.w = w;
+    P.h = h;
+    for (uint32_t i = 0; i < n; ++i) gm[i] = (uint32_t)camera_ray(&P, px[i], py[i], ro + 3 * i, rd + 3 * i);
+}
+
+typedef struct {
+    const oparams *P;
+    uint32_t x0, y0, rx, ry, tilex, samples, sample_base;
+    float *out;              /* 4 buffers of tilex*tiley*4: lights, env, sumsq lights, sumsq env */
+    size_t plane;
+    uint32_t next_row;
+    pthread_mutex_t mu;
+} ojob;
+
+static void render_pixel(ojob *J, octx *X, uint32_t lx, uint32_t ly) {
+    const oparams *P = J->P;
+    uint32_t px = J->x0 + lx, py = J->y0 + ly;
+    float ro[3], rd[3];
+    int gm = camera_ray(P, px, py, ro, rd);
+    size_t o = ((size_t)ly * J->tilex + lx) * 4;
+    if (!gm) return;
+    X->key0 = py * P->w + px;
+    float depth = P->depth ? P->depth[X->key0] : FLT_MAX;
+    double acc[4][4] = {{0}};
+    for (uint32_t s = 0; s < J->samples; ++s) {
+        X->sample = J->sample_base + s;
+        ores R;
+        sample_volume(X, ro, rd, 0, 0u, depth, &R);
+        for (int k = 0; k < 4; ++k) {
+            acc[0][k] += R.cl[k];
+            acc[1][k] += R.cs[k];
+            acc[2][k] += k < 3 ? (double)R.cl[k] * R.cl[k] : R.cl[k];
+            acc[3][k] += k < 3 ? (double)R.cs[k] * R.cs[k] : R.cs[k];
+        }
+    }
+    for (int b = 0; b < 4; ++b)
+        for (int k = 0; k < 4; ++k) J->out[b * J->plane + o + k] = (float)acc[b][k];
+}
+
+static void *render_worker(void *arg) {
+    ojob *J = (ojob *)arg;
+    octx X;
+    memset(&X, 0, sizeof(X));
+    X.P = J->P;
+    X.L.ob = J->P->ob;
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        uint32_t row = J->next_row++;
+        pthread_mutex_unlock(&J->mu);
+        if (row >= J->ry) break;
+        for (uint32_t lx = 0; lx < J->rx; ++lx) render_pixel(J, &X, lx, row);
+    }
+    free(X.L.ls);
+    return NULL;
+}
+
+/*
+ * Renders tile (x0,y0,tilex,tiley) with `samples` paths per pixel into out
+ * (4 * tilex*tiley*4 floats: lights RGBA sums, environment RGBA sums, and the
+ * per-pixel sums of squares of the lights / environment RGB, alpha repeated).
+ * Pixels of the dropped odd row/column stay 0. Uses `threads` POSIX threads.
+ * Parameter block layout matches oparams above (filled by pyoracle.py).
+ */
+int or_render(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+              uint32_t sample_base, uint32_t threads, float *out) {
+    ojob J;
+    memset(&J, 0, sizeof(J));
+    J.P = P;
+    J.x0 = x0; J.y0 = y0;
+    J.rx = tilex & ~1u; J.ry = tiley & ~1u;
+    if (x0 + J.rx > P->w) J.rx = x0 < P->w ? P->w - x0 : 0;
+    if (y0 + J.ry > P->h) J.ry = y0 < P->h ? P->h - y0 : 0;
+    J.tilex = tilex;
+    J.samples = samples;
+    J.sample_base = sample_base;
+    J.out = out;
+    J.plane = (size_t)tilex * tiley * 4;
+    memset(out, 0, sizeof(float) * 4 * J.plane);
+    pthread_mutex_init(&J.mu, NULL);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    for (uint32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, render_worker, &J);
+    for (uint32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&J.mu);
+    return 0;
+}
+
+uint32_t or_params_size(void) { return (uint32_t)sizeof(oparams); }
+uint32_t or_mie_size(void) { return (uint32_t)sizeof(omie); }

@@ -78,6 +78,8 @@ _SIGS = {
                                     u32, C.POINTER(_P)]),
     "drk_kernel_set_seed": (C.c_int, [_P, u64, u32]),
     "drk_kernel_set_tuning": (C.c_int, [_P, u32, u32]),
+    "drk_kernel_set_timing": (C.c_int, [_P, C.c_int]),
+    "drk_kernel_timing": (C.c_int, [_P, C.POINTER(C.c_double), _U]),
     "drk_kernel_render": (C.c_int, [_P, u32, u32, u32, u32, u32]),
     "drk_kernel_shadow": (C.c_int, [_P, u32, u32, u32, u32, u32]),
     "drk_kernel_phb": (_F, [_P, u32]),

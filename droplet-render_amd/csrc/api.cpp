@@ -73,7 +73,9 @@ struct drk_kernel {
     uint32_t blocks_per_cu = 1;
     uint32_t chunk_override = 0;
     uint32_t grid_override = 0;
-    unsigned long long last_counts[2] = {0, 0};
+    // Live kernel timing: event pairs recorded around each k_render launch.
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_used;
 };
 
 extern "C" {
@@ -338,20 +340,22 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     P.key1 = rng_key1(seed);
     P.depth = k->d_depth;
     const bool sumsq = d_sql && d_sqe;
-    // Pixels outside the rendered extent (odd tiles) or outside the image keep
-    // zeros; the reference leaves them untouched.
-    size_t tile_n = (size_t)tilex * tiley * 4;
-    HIP_TRY(launch_fill(d_l, tile_n, 0.0f, s), "render: clear");
-    HIP_TRY(launch_fill(d_e, tile_n, 0.0f, s), "render: clear");
-    if (sumsq) {
-        HIP_TRY(launch_fill(d_sql, tile_n, 0.0f, s), "render: clear");
-        HIP_TRY(launch_fill(d_sqe, tile_n, 0.0f, s), "render: clear");
-    }
     // Clip the rendered extent to the image (gm mask, src/kernel.cpp:472-474).
-    if (P.x0 >= P.w || P.y0 >= P.h) return DRK_OK;
-    uint32_t rx = std::min(P.rx, P.w - P.x0), ry = std::min(P.ry, P.h - P.y0);
+    uint32_t rx = P.x0 < P.w ? std::min(P.rx, P.w - P.x0) : 0u;
+    uint32_t ry = P.y0 < P.h ? std::min(P.ry, P.h - P.y0) : 0u;
     P.rx = rx;
     P.ry = ry;
+    // Pixels outside the rendered extent (odd tiles, or outside the image)
+    // are zeroed; the reference leaves them untouched.
+    if (rx != tilex || ry != tiley) {
+        size_t tile_n = (size_t)tilex * tiley * 4;
+        HIP_TRY(launch_fill(d_l, tile_n, 0.0f, s), "render: clear");
+        HIP_TRY(launch_fill(d_e, tile_n, 0.0f, s), "render: clear");
+        if (sumsq) {
+            HIP_TRY(launch_fill(d_sql, tile_n, 0.0f, s), "render: clear");
+            HIP_TRY(launch_fill(d_sqe, tile_n, 0.0f, s), "render: clear");
+        }
+    }
     if (rx == 0 || ry == 0) return DRK_OK;
 
     const uint32_t npix = rx * ry;
@@ -393,7 +397,19 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         W.partial = k->d_partial;
     }
     if (count) HIP_TRY(hipMemsetAsync(k->d_counts, 0, 16, s), "render: counters");
-    HIP_TRY(launch_render(P, W, grid, sumsq, count, s), "render: launch");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (k->timing && !count) {
+        if (k->ev_free.empty()) {
+            HIP_TRY(hipEventCreate(&e0), "render: event");
+            HIP_TRY(hipEventCreate(&e1), "render: event");
+        } else {
+            e0 = k->ev_free.back().first;
+            e1 = k->ev_free.back().second;
+            k->ev_free.pop_back();
+        }
+        k->ev_used.emplace_back(e0, e1);
+    }
+    HIP_TRY(launch_render(P, W, grid, sumsq, count, s, e0, e1), "render: launch");
     return DRK_OK;
 }
 
@@ -444,6 +460,28 @@ int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
     return set_error(DRK_EINVAL, "drk_kernel_shadow: the Shadow pass is not implemented yet");
 }
 
+int drk_kernel_set_timing(drk_kernel *k, int enable) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_timing: NULL kernel");
+    k->timing = enable != 0;
+    return DRK_OK;
+}
+
+int drk_kernel_timing(drk_kernel *k, double *total_ms, uint32_t *launches) {
+    if (!k || !total_ms || !launches) return set_error(DRK_EINVAL, "drk_kernel_timing: NULL argument");
+    double t = 0.0;
+    for (auto &pr : k->ev_used) {
+        HIP_TRY(hipEventSynchronize(pr.second), "drk_kernel_timing: sync");
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second), "drk_kernel_timing: elapsed");
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = (uint32_t)k->ev_used.size();
+    for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
+    k->ev_used.clear();
+    return DRK_OK;
+}
+
 const float *drk_kernel_phb(const drk_kernel *k, uint32_t buffer) {
     if (!k || buffer > 3) {
         set_error(DRK_EINVAL, "drk_kernel_phb: bad kernel or buffer index");
@@ -461,6 +499,11 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_counts) (void)hipFree(k->d_counts);
     if (k->d_env) (void)hipFree(k->d_env);
     if (k->d_depth) (void)hipFree(k->d_depth);
+    for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
+    for (auto &pr : k->ev_free) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
     if (k->stream) (void)hipStreamDestroy(k->stream);
     std::free(k->h_phb);
     delete k;

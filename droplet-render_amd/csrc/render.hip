@@ -371,13 +371,15 @@ int render_blocks_per_cu(bool sumsq, bool count) {
 }
 
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, bool sumsq, bool count,
-                         hipStream_t s) {
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     k_fill_u32<<<1, 64, 0, s>>>(W.counter, 1, 0u);
+    if (ev0) (void)hipEventRecord(ev0, s);
     if (count) k_render<true, true><<<grid, kBlock, 0, s>>>(P, W);
     else if (sumsq) k_render<true, false><<<grid, kBlock, 0, s>>>(P, W);
     else k_render<false, false><<<grid, kBlock, 0, s>>>(P, W);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (ev1) (void)hipEventRecord(ev1, s);
     if (W.partial) {
         uint32_t nb = (W.npix + 255) / 256;
         k_reduce<<<nb, 256, 0, s>>>(W, P.rx, P.tilex, sumsq ? 4u : 2u);

@@ -23,8 +23,9 @@ struct WorkParams {
 
 int render_block_size();
 int render_blocks_per_cu(bool sumsq, bool count);
+// ev0/ev1 (may be null) are recorded on s right before and after k_render.
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, bool sumsq, bool count,
-                         hipStream_t s);
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 
 }  // namespace drk

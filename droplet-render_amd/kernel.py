@@ -49,6 +49,15 @@ class RenderKernel:
     def set_tuning(self, chunk=0, grid_blocks=0):
         L.check(L.lib().drk_kernel_set_tuning(self._h, int(chunk), int(grid_blocks)))
 
+    def set_timing(self, enable=True):
+        L.check(L.lib().drk_kernel_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """(total ms, launches) of the path-tracing kernel since the last call."""
+        ms, n = C.c_double(), C.c_uint32()
+        L.check(L.lib().drk_kernel_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def Render(self, x0, y0, tilex, tiley, samples):
         """RenderKernel::Render (src/kernel.cpp:536-567); blocks until phb holds the sums."""
         L.check(L.lib().drk_kernel_render(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples)))

@@ -178,6 +178,13 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
  * counter-based, so the counts describe any run with the same arguments.
  */
 int drk_kernel_set_tuning(drk_kernel *k, uint32_t chunk, uint32_t grid_blocks);
+
+/* Live kernel timing: when enabled, HIP events are recorded on the launch
+ * stream right around every path-tracing kernel launch. drk_kernel_timing
+ * waits for them, returns the summed duration (ms) and launch count since the
+ * last query, and resets. */
+int drk_kernel_set_timing(drk_kernel *k, int enable);
+int drk_kernel_timing(drk_kernel *k, double *total_ms, uint32_t *launches);
 int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
                      uint32_t samples, uint64_t seed, uint32_t sample_base,
                      unsigned long long counts[2]);

@@ -209,8 +209,9 @@ def camera_rays(viewi, proji, w, h, px, py):
 
 
 # --------------------------------------------------------------- renderer --
-def make_params(wl, seed):
-    """Fills the oracle's parameter block from a droplet_render_amd Workload."""
+def make_params(wl, seed, depth=None):
+    """Fills the oracle's parameter block from a droplet_render_amd Workload
+    (depth: optional w*h compositing depth)."""
     P = OParams()
     sc = wl.scene
     keep = [np.ascontiguousarray(sc.ob), f32(sc.pvol[1])]
@@ -241,18 +242,22 @@ def make_params(wl, seed):
         P.sky.rad[:] = [float(x) for x in f32(rad)]
         P.sky.dir[:] = [float(x) for x in d0]
     P.mie = C.pointer(_MIE)
-    P.depth = None
+    if depth is not None:
+        keep.append(f32(depth).reshape(-1))
+        P.depth = fp(keep[-1])
+    else:
+        P.depth = None
     P.key1 = rng_key1(seed)
     return P, keep
 
 
-def render(wl, seed, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None, threads=None):
+def render(wl, seed, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None, threads=None, depth=None):
     """CPU render of a tile; returns [lights, env, sumsq lights, sumsq env] as (tiley, tilex, 4) float32."""
     tilex = tilex or wl.w
     tiley = tiley or wl.h
     samples = samples or wl.spp
     threads = threads or os.cpu_count() or 1
-    P, keep = make_params(wl, seed)
+    P, keep = make_params(wl, seed, depth)
     out = np.zeros((4, tiley, tilex, 4), np.float32)
     lib().or_render(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
                     C.c_uint32(samples), C.c_uint32(sample_base), C.c_uint32(threads), fp(out))

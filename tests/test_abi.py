@@ -1,0 +1,93 @@
+"""C ABI: the library loads, exports every declared entry point, and rejects
+bad arguments with error codes before touching a device."""
+import ctypes as C
+import glob
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(drk_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_every_declared_symbol_is_exported(pkg):
+    lib = pkg._lib.LIB_PATH
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], stdout=subprocess.PIPE, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(drk_\w+)", out))
+    missing = declared_symbols() - exported
+    assert not missing, missing
+    # and the Python binding knows every declared symbol
+    assert declared_symbols() <= set(pkg._lib.EXPORTED)
+
+
+def test_library_loads_and_binds(pkg):
+    L = pkg._lib.lib()
+    for name in pkg._lib.EXPORTED:
+        assert getattr(L, name) is not None
+
+
+def test_node_record_layout(pkg):
+    N = pkg._lib.OctreeNode
+    assert C.sizeof(N) == 64
+    assert N.ce.offset == 0 and N.chn.offset == 16 and N.volx.offset == 48 and N.qval.offset == 56
+    assert pkg.Scene.node_dtype.itemsize == 64
+
+
+def test_host_argument_errors(pkg):
+    L = pkg._lib.lib()
+    out = C.POINTER(pkg._lib.HostScene)()
+    g = np.zeros((8, 8, 8), np.float32)
+    assert L.drk_scene_build_dense(None, None, 8, 8, 8, 0.1, 12, C.byref(out)) == pkg._lib.DRK_EINVAL
+    assert b"invalid" in L.drk_last_error()
+    assert L.drk_scene_build_dense(pkg._lib.fptr(g), None, 8, 8, 8, -1.0, 12, C.byref(out)) == pkg._lib.DRK_EINVAL
+    # an all-empty grid has no leaves
+    assert L.drk_scene_build_dense(pkg._lib.fptr(g), None, 8, 8, 8, 0.1, 12, C.byref(out)) == pkg._lib.DRK_EINVAL
+    prm = np.zeros(9, np.float32)
+    assert L.drk_synth_cloud(7, 8, 1, pkg._lib.fptr(prm), 9, pkg._lib.fptr(g)) == pkg._lib.DRK_EINVAL
+    h = C.c_void_p()
+    assert L.drk_scene_create(None, 0, None, 0, None, 0, 8, C.byref(h)) == pkg._lib.DRK_EINVAL
+    assert L.drk_kernel_create(None, None, None, None, None, 1, 1.0, 0.0, 8, 8, 8, 8, 0, C.byref(h)) == pkg._lib.DRK_EINVAL
+    assert L.drk_kernel_render(None, 0, 0, 8, 8, 1) == pkg._lib.DRK_EINVAL
+    assert L.drk_kernel_phb(None, 0) is None or not bool(L.drk_kernel_phb(None, 0))
+
+
+def test_scene_create_rejects_bad_octrees_on_host(pkg):
+    """Structural checks run before any device allocation."""
+    L = pkg._lib.lib()
+    g = pkg.configs.synth_fbm(32, 3, octaves=3, freq=3.0, threshold=0.2)
+    sc = pkg.Scene.from_grid(g)
+    h = C.c_void_p()
+
+    def create(ob, pvol, leafx):
+        return L.drk_scene_create(ob.ctypes.data_as(C.POINTER(pkg._lib.OctreeNode)), len(ob), None, 0,
+                                  pkg._lib.fptr(pvol), leafx, sc.lvoxc, C.byref(h))
+    bad = sc.ob.copy()
+    bad["chn"][0, 0] = len(bad) + 5            # child out of range
+    assert create(bad, sc.pvol[1], sc.leafx[1]) == pkg._lib.DRK_EINVAL
+    bad = sc.ob.copy()
+    bad["chn"][2, 0] = 1                       # child index not above its parent (cycle)
+    assert create(bad, sc.pvol[1], sc.leafx[1]) == pkg._lib.DRK_EINVAL
+    bad = sc.ob.copy()
+    leaf = int(np.nonzero(bad["volx"][:, 1] != 0xFFFFFFFF)[0][0])
+    bad["volx"][leaf, 1] = sc.leafx[1]         # brick index out of range
+    assert create(bad, sc.pvol[1], sc.leafx[1]) == pkg._lib.DRK_EINVAL
+    bad = sc.ob.copy()
+    bad["volx"][leaf, 0] = 0                   # SDF bricks are not accepted
+    assert create(bad, sc.pvol[1], sc.leafx[1]) == pkg._lib.DRK_EINVAL
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True)
+    assert r.returncode != 0 and "one GPU" in r.stderr

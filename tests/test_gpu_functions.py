@@ -1,0 +1,157 @@
+"""Function-level parity of the gfx950 device code against the CPU oracle.
+
+Each test runs one device function through the C-ABI probe hooks. Integer and
+index work (RNG, traversal, trilinear lookup, camera rays) must be bit-exact;
+floating-point point functions (phase, sky) are held to 1e-5 relative against
+the FP64 restatement, as the accuracy bar for FP32 requires.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SUN = np.array([math.cos(math.radians(30)), 0.0, math.sin(math.radians(30))], np.float32)
+
+
+def test_philox_bit_exact(pkg, oracle, gpu):
+    rows = []
+    for line in open(os.path.join(ROOT, "tests", "golden", "philox_kat.txt")):
+        a, b = line.split("->")
+        v = [int(x, 16) for x in a.split()]
+        rows.append((v[:4], v[4:], [int(x, 16) for x in b.split()]))
+    ctr = np.array([r[0] for r in rows], np.uint32)
+    key = np.array([r[1] for r in rows], np.uint32)
+    assert np.array_equal(pkg.probe.philox(ctr, key), np.array([r[2] for r in rows], np.uint32))
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2 ** 32, (4096, 4), dtype=np.uint64).astype(np.uint32)
+    key = rng.integers(0, 2 ** 32, (4096, 2), dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(pkg.probe.philox(ctr, key), oracle.philox(ctr, key))
+
+
+def test_ray_streams_bit_exact(pkg, oracle, gpu):
+    rng = np.random.default_rng(1)
+    n = 2048
+    k0 = rng.integers(0, 2 ** 20, n).astype(np.uint32)
+    s = rng.integers(0, 4096, n).astype(np.uint32)
+    r = rng.integers(0, 129, n).astype(np.uint32)
+    assert np.array_equal(pkg.probe.ray_stream(k0, s, r, 12345, 64), oracle.ray_stream(k0, s, r, 12345, 64))
+
+
+def test_camera_rays_bit_exact(pkg, oracle, gpu):
+    wl = pkg.configs.tiny(w=96, h=64)
+    ys, xs = np.mgrid[0:64, 0:96]
+    g = pkg.probe.camera(wl.viewi, wl.proji, 96, 64, xs.ravel(), ys.ravel())
+    c = oracle.camera_rays(wl.viewi, wl.proji, 96, 64, xs.ravel(), ys.ravel())
+    for a, b in zip(g, c):
+        assert np.array_equal(a, b)
+
+
+def test_mie_eval_within_1e5(pkg, oracle, gpu):
+    from oracle import fp64
+    rng = np.random.default_rng(2)
+    ct = np.concatenate([rng.uniform(-1, 1, 20000), [-1, 1, 0, 1.2, -1.2]]).astype(np.float32)
+    g = pkg.probe.mie_eval(ct).astype(np.float64)
+    ref = fp64.mie_eval(ct.astype(np.float64))
+    rel = np.abs(g - ref) / np.abs(ref)
+    assert np.quantile(rel, 0.999) < 1e-5 and rel.max() < 1e-3, np.quantile(rel, [0.5, 0.999, 1])
+    c = oracle.mie_eval(ct).astype(np.float64)
+    assert np.quantile(np.abs(g - c) / np.abs(c), 0.999) < 1e-5
+
+
+def test_mie_sample_same_bin_as_reference_scan(pkg, oracle, gpu):
+    """The kernels' binary search picks the same quantised angle as the
+    reference's linear scan for every u."""
+    rng = np.random.default_rng(3)
+    n = 50000
+    u1 = ((rng.integers(0, 1 << 23, n) * 2 + 1) * 2.0 ** -24).astype(np.float32)
+    u2 = rng.random(n).astype(np.float32)
+    iv = np.tile(np.array([[0.48, 0.6, 0.64]], np.float32), (n, 1))
+    iv /= np.linalg.norm(iv, axis=1, keepdims=True)
+    d = pkg.probe.mie_sample(iv, u1, u2).astype(np.float64)
+    th = oracle.mie_theta(u1).astype(np.float64) * np.pi
+    assert np.allclose((d * iv).sum(axis=1), -np.cos(th), atol=3e-6)
+    c = oracle.mie_sample(iv[:4000], u1[:4000], u2[:4000]).astype(np.float64)
+    assert np.allclose(d[:4000], c, atol=3e-6)
+
+
+@pytest.mark.parametrize("g", [0.75, -0.4])
+def test_hg_within_1e5(pkg, oracle, gpu, g):
+    from oracle import fp64
+    ct = np.linspace(-1, 1, 4001).astype(np.float32)
+    e = pkg.probe.hg_eval(g, ct).astype(np.float64)
+    assert np.max(np.abs(e - fp64.hg_eval(g, ct)) / fp64.hg_eval(g, ct)) < 1e-5
+    rng = np.random.default_rng(4)
+    iv = rng.normal(size=(2000, 3)).astype(np.float32)
+    iv /= np.linalg.norm(iv, axis=1, keepdims=True)
+    u1, u2 = rng.random(2000).astype(np.float32), rng.random(2000).astype(np.float32)
+    assert np.allclose(pkg.probe.hg_sample(g, iv, u1, u2), oracle.hg_sample(g, iv, u1, u2), atol=3e-6)
+
+
+def test_sun_matches_oracle(pkg, oracle, gpu):
+    rng = np.random.default_rng(5)
+    n = 4000
+    u1, u2 = rng.random(n).astype(np.float32), rng.random(n).astype(np.float32)
+    dirs = np.concatenate([np.tile(SUN, (n // 2, 1)), rng.normal(size=(n - n // 2, 3))]).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    g = pkg.probe.sun(SUN, (1, 0.9, 0.8), 0.01, u1, u2, dirs)
+    c = oracle.sun(SUN, (1, 0.9, 0.8), 0.01, u1, u2, dirs)
+    assert np.allclose(g[0], c[0], atol=3e-6)
+    assert np.array_equal(g[1], c[1]) and np.array_equal(g[2], c[2])
+
+
+def test_sky_within_1e5_of_fp64(pkg, gpu):
+    from oracle import fp64
+    rng = np.random.default_rng(6)
+    rd = rng.normal(size=(20000, 3))
+    rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    rd = rd[np.abs(rd[:, 1]) > 1e-3].astype(np.float32)
+    g = pkg.probe.sky(SUN, 3.0, 0.1, rd).astype(np.float64)
+    ref = fp64.sky(SUN, 3.0, 0.1, rd.astype(np.float64))
+    ok = np.isfinite(ref).all(axis=1)
+    rel = np.abs(g[ok] - ref[ok]) / np.abs(ref[ok])
+    assert np.quantile(rel, 0.99) < 1e-5 and rel.max() < 2e-4, np.quantile(rel, [0.5, 0.99, 1])
+    assert np.array_equal(np.isfinite(g), np.isfinite(ref))
+
+
+def test_trilinear_bit_exact(pkg, oracle, gpu):
+    rng = np.random.default_rng(7)
+    for lvoxc in (8, 16):
+        brick = rng.random(lvoxc ** 3).astype(np.float32)
+        ce = np.array([0.41, 0.23, -0.6, 0.0625], np.float32)
+        p = (ce[:3] - ce[3] * 1.01 + rng.random((5000, 3)) * 2.02 * ce[3]).astype(np.float32)
+        assert np.array_equal(pkg.probe.trilinear(brick, lvoxc, ce, p), oracle.voxel(brick, p, ce, lvoxc))
+
+
+@pytest.mark.parametrize("cfg", ["c0", "spheres64"])
+def test_traversal_bit_exact(pkg, oracle, gpu, cfg):
+    """Leaf sequences (node indices and [tr0, tr1]) of camera rays and of
+    secondary rays starting inside the volume, against the oracle's recursive
+    full traverser."""
+    if cfg == "c0":
+        wl = pkg.configs.c0(w=64, h=64)
+        sc = wl.scene
+        ys, xs = np.mgrid[0:64, 0:64]
+        ro, rd, _ = oracle.camera_rays(wl.viewi, wl.proji, 64, 64, xs.ravel(), ys.ravel())
+    else:
+        sc = pkg.Scene.from_grid(pkg.configs.synth_spheres(64, 3, spheres=12))
+        ro = np.zeros((0, 3), np.float32)
+        rd = np.zeros((0, 3), np.float32)
+    rng = np.random.default_rng(8)
+    m = 6000
+    ro2 = rng.uniform(0.0, 1.0, (m, 3)).astype(np.float32)
+    rd2 = rng.normal(size=(m, 3))
+    rd2 /= np.linalg.norm(rd2, axis=1, keepdims=True)
+    # axis-aligned and diagonal directions exercise the tie rules
+    rd2[:200] = np.array([[1, 0, 0], [0, -1, 0], [0, 0, 1], [-1, 0, 0]] * 50)
+    rd2[200:300] = np.array([1, 1, 1]) / math.sqrt(3)
+    ro = np.concatenate([ro, ro2]).astype(np.float32)
+    rd = np.concatenate([rd, rd2]).astype(np.float32)
+    g = pkg.probe.traverse(sc, ro, rd)
+    c = oracle.traverse(sc.ob, ro, rd)
+    assert np.array_equal(g[3], c[3])
+    for a, b in zip(g[:3], c[:3]):
+        assert np.array_equal(a, b)
+    assert g[3].max() > 3

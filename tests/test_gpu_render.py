@@ -39,8 +39,10 @@ def same_key_agreement(g, c, min_close=0.99):
 
 
 def three_sigma(g, c, n_g, n_c):
-    """Per-pixel z of the combined image (lights + environment RGB)."""
-    zs = []
+    """Per-pixel z for each (buffer, channel) image; pixels are independent
+    within one image (channels of one pixel share paths, so each image is
+    tested on its own)."""
+    worst = 0.0
     for b in range(2):
         for ch in range(3):
             mg = g[b][..., ch].astype(np.float64) / n_g
@@ -49,14 +51,14 @@ def three_sigma(g, c, n_g, n_c):
             vc = np.maximum(c[2 + b][..., ch] / n_c - mc ** 2, 0) / n_c
             se = np.sqrt(vg + vc)
             m = se > 0
-            zs.append(((mg - mc)[m] / se[m]))
-    z = np.concatenate(zs)
-    n = len(z)
-    out_frac = np.mean(np.abs(z) > 3)
-    bound = 0.0027 + 4 * math.sqrt(0.0027 * 0.9973 / max(n, 1))
-    assert out_frac <= bound, (out_frac, bound, n)
-    assert abs(z.mean()) < 4 / math.sqrt(n), z.mean()
-    return out_frac
+            z = (mg - mc)[m] / se[m]
+            n = len(z)
+            out_frac = np.mean(np.abs(z) > 3)
+            bound = 0.0027 + 4 * math.sqrt(0.0027 * 0.9973 / max(n, 1))
+            assert out_frac <= bound, (b, ch, out_frac, bound, n)
+            assert abs(z.mean()) < 4 / math.sqrt(n), (b, ch, z.mean(), n)
+            worst = max(worst, out_frac)
+    return worst
 
 
 def test_tiny_same_keys(pkg, oracle, gpu):
@@ -128,9 +130,12 @@ def test_hg_nullenv_and_mapenv(pkg, oracle, gpu):
     g = render_gpu(wl, 21)
     same_key_agreement(g, oracle.render(wl, seed=21))
     assert not g[1][..., :3].any()
-    rng = np.random.default_rng(0)
-    tex = rng.random((16, 32, 4)).astype(np.float32)
-    wl.env = pkg.MapEnv(tex)
+    # a smooth sky-like map (a white-noise map turns 1-ulp direction
+    # differences into large radiance differences)
+    th, ph = np.mgrid[0:16, 0:32]
+    tex = np.stack([0.3 + 0.2 * np.cos(np.pi * th / 15), 0.4 + 0.1 * np.sin(2 * np.pi * ph / 31),
+                    0.6 + 0.3 * np.cos(np.pi * th / 15) * np.cos(2 * np.pi * ph / 31), np.ones_like(th)], -1)
+    wl.env = pkg.MapEnv(tex.astype(np.float32))
     g = render_gpu(wl, 22)
     same_key_agreement(g, oracle.render(wl, seed=22))
     assert g[1][..., :3].sum() > 0

@@ -1,0 +1,57 @@
+"""torch.autograd wrapper of the render and its adjoint.
+
+`render(kernel, density, ...)` returns the two phb sum images for FOG brick
+values `density` (a CUDA tensor in the layout of Scene.pvol[FOG]); its
+backward runs the path-replay adjoint kernel (drk_kernel_adjoint_device) with
+the same RNG keys, so d(image)/d(density) is differentiated for the very
+samples the forward drew. The octree's majorants stay fixed, as the adjoint
+assumes: keep the densities at or below them (`Scene.raise_majorants`).
+"""
+import torch
+
+from . import _lib as L
+
+
+class _Render(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, density, kernel, x0, y0, tilex, tiley, samples, seed, sample_base):
+        scene = kernel.scene
+        assert density.is_cuda and density.dtype == torch.float32 and density.is_contiguous()
+        assert density.numel() == scene.leafx[1] * scene.lvoxc3
+        torch.cuda.current_stream(density.device).synchronize()
+        scene.update_fog(density)                        # device-to-device copy into the scene
+        n = tilex * tiley * 4
+        out_l = torch.empty(n, dtype=torch.float32, device=density.device)
+        out_e = torch.empty(n, dtype=torch.float32, device=density.device)
+        stream = torch.cuda.current_stream(density.device)
+        kernel.render_device(x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, stream=stream)
+        ctx.kernel = kernel
+        ctx.args = (x0, y0, tilex, tiley, samples, seed, sample_base)
+        ctx.save_for_backward(density)
+        return out_l.view(tiley, tilex, 4), out_e.view(tiley, tilex, 4)
+
+    @staticmethod
+    def backward(ctx, g_l, g_e):
+        (density,) = ctx.saved_tensors
+        kernel = ctx.kernel
+        x0, y0, tilex, tiley, samples, seed, sample_base = ctx.args
+        torch.cuda.current_stream(density.device).synchronize()
+        kernel.scene.update_fog(density)                 # the adjoint replays these densities
+        g_l = torch.zeros(tiley, tilex, 4, device=density.device) if g_l is None else g_l.contiguous().float()
+        g_e = torch.zeros(tiley, tilex, 4, device=density.device) if g_e is None else g_e.contiguous().float()
+        grad = torch.zeros_like(density)
+        stream = torch.cuda.current_stream(density.device)
+        kernel.adjoint_device(x0, y0, tilex, tiley, samples, seed, sample_base, g_l, g_e, grad, stream=stream)
+        return grad, None, None, None, None, None, None, None, None
+
+
+def render(kernel, density, samples=None, seed=0, sample_base=0, x0=0, y0=0, tilex=None, tiley=None):
+    """Differentiable render: (lights sums, environment sums), each (tiley, tilex, 4)."""
+    tilex = tilex or kernel.tilex
+    tiley = tiley or kernel.tiley
+    samples = samples or 1
+    return _Render.apply(density, kernel, int(x0), int(y0), int(tilex), int(tiley), int(samples), int(seed),
+                         int(sample_base))
+
+
+__all__ = ["render", "L"]

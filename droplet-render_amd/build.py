@@ -60,6 +60,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    # Test driver of the C++ RenderKernel facade (host code only, plain g++).
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"),
+          os.path.join(ROOT, "tests", "cpp", "facade_render.cpp"), "-o", os.path.join(OBJDIR, "facade_render"),
+          "-L" + HERE, "-ldroplet_amd", "-Wl,-rpath," + HERE])
     return LIB
 
 

@@ -73,6 +73,7 @@ struct drk_kernel {
     uint32_t blocks_per_cu = 1;
     uint32_t chunk_override = 0;
     uint32_t grid_override = 0;
+    bool depthcomp = false;
     // Live kernel timing: event pairs recorded around each k_render launch.
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_used;
@@ -282,7 +283,15 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
         P.env_w = smp->env_w;
         P.env_h = smp->env_h;
     }
-    if (e == hipSuccess && pdepth && (flags & DRK_KERNEL_DEPTHCOMP)) {
+    // The depth map serves compositing (flag) and the Shadow pass (always).
+    k->depthcomp = (flags & DRK_KERNEL_DEPTHCOMP) != 0;
+    {
+        // Far plane: -z/w of (0,0,1,1)*proji (src/kernel.cpp:574-575).
+        float zf_z = 0.0f * proji[0 * 4 + 2] + 0.0f * proji[1 * 4 + 2] + 1.0f * proji[2 * 4 + 2] + 1.0f * proji[3 * 4 + 2];
+        float zf_w = 0.0f * proji[0 * 4 + 3] + 0.0f * proji[1 * 4 + 3] + 1.0f * proji[2 * 4 + 3] + 1.0f * proji[3 * 4 + 3];
+        P.clip_end = -zf_z / zf_w;
+    }
+    if (e == hipSuccess && pdepth) {
         e = hipMalloc(&k->d_depth, (size_t)w * h * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(k->d_depth, pdepth, (size_t)w * h * sizeof(float), hipMemcpyHostToDevice);
     }
@@ -299,7 +308,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     uint32_t cus = 0;
     if (drk_device_info(nullptr, 0, &cus) != DRK_OK || cus == 0) cus = 1;
     k->cu_count = cus;
-    k->blocks_per_cu = (uint32_t)render_blocks_per_cu(true, false);
+    k->blocks_per_cu = (uint32_t)render_blocks_per_cu(kModeForwardSq);
     if (const char *c = std::getenv("DRK_CHUNK")) k->chunk_override = (uint32_t)std::atoi(c);
     if (const char *g = std::getenv("DRK_GRID")) k->grid_override = (uint32_t)std::atoi(g);
     *out = k;
@@ -320,9 +329,15 @@ int drk_kernel_set_tuning(drk_kernel *k, uint32_t chunk, uint32_t grid_blocks) {
     return DRK_OK;
 }
 
+// Adjoint-mode inputs of render_impl.
+struct AdjointArgs {
+    const float *adj_l = nullptr, *adj_e = nullptr;
+    float *grad = nullptr;
+};
+
 static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
-                       hipStream_t s, bool count) {
+                       hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr) {
     if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
         return set_error(DRK_EINVAL, "render: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley, k->tilex, k->tiley);
     if (samples == 0) return set_error(DRK_EINVAL, "render: samples must be > 0");
@@ -338,8 +353,10 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     P.samples = samples;
     P.sample_base = sample_base;
     P.key1 = rng_key1(seed);
-    P.depth = k->d_depth;
+    P.shadow_mode = shadow ? 1u : 0u;
+    P.depth = (shadow || k->depthcomp) ? k->d_depth : nullptr;
     const bool sumsq = d_sql && d_sqe;
+    const int mode = adj ? kModeAdjoint : count ? kModeCount : sumsq ? kModeForwardSq : kModeForward;
     // Clip the rendered extent to the image (gm mask, src/kernel.cpp:472-474).
     uint32_t rx = P.x0 < P.w ? std::min(P.rx, P.w - P.x0) : 0u;
     uint32_t ry = P.y0 < P.h ? std::min(P.ry, P.h - P.y0) : 0u;
@@ -347,7 +364,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     P.ry = ry;
     // Pixels outside the rendered extent (odd tiles, or outside the image)
     // are zeroed; the reference leaves them untouched.
-    if (rx != tilex || ry != tiley) {
+    if (!adj && (rx != tilex || ry != tiley)) {
         size_t tile_n = (size_t)tilex * tiley * 4;
         HIP_TRY(launch_fill(d_l, tile_n, 0.0f, s), "render: clear");
         HIP_TRY(launch_fill(d_e, tile_n, 0.0f, s), "render: clear");
@@ -359,7 +376,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     if (rx == 0 || ry == 0) return DRK_OK;
 
     const uint32_t npix = rx * ry;
-    const uint32_t bpc = count ? (uint32_t)render_blocks_per_cu(true, true) : k->blocks_per_cu;
+    const uint32_t bpc = mode == kModeForwardSq ? k->blocks_per_cu : (uint32_t)render_blocks_per_cu(mode);
     uint32_t grid = k->grid_override ? k->grid_override : k->cu_count * std::max(1u, bpc);
     const uint64_t lanes = (uint64_t)grid * render_block_size();
     uint32_t chunk = k->chunk_override;
@@ -385,7 +402,12 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     W.sq_l = sumsq ? d_sql : nullptr;
     W.sq_e = sumsq ? d_sqe : nullptr;
     W.counts = k->d_counts;
-    if (nchunks > 1) {
+    if (adj) {
+        W.adj_l = adj->adj_l;
+        W.adj_e = adj->adj_e;
+        W.grad = adj->grad;
+    }
+    if (nchunks > 1 && !adj) {
         size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
         if (need > k->partial_bytes) {
             if (k->d_partial) (void)hipFree(k->d_partial);
@@ -409,7 +431,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         }
         k->ev_used.emplace_back(e0, e1);
     }
-    HIP_TRY(launch_render(P, W, grid, sumsq, count, s, e0, e1), "render: launch");
+    HIP_TRY(launch_render(P, W, grid, mode, s, e0, e1), "render: launch");
     return DRK_OK;
 }
 
@@ -454,10 +476,64 @@ int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, ui
     return DRK_OK;
 }
 
+int drk_kernel_adjoint_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                              uint32_t samples, uint64_t seed, uint32_t sample_base, const float *d_adj_lights,
+                              const float *d_adj_env, float *d_grad_fog, void *stream) {
+    if (!k || !d_adj_lights || !d_adj_env || !d_grad_fog)
+        return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: NULL argument");
+    if (k->P.scattevs > 64) return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: scattevs > 64 is not supported");
+    if (k->depthcomp) return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: depth compositing is not differentiated");
+    if (k->scene->leafx[1] == 0) return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: scene has no FOG bricks");
+    AdjointArgs A;
+    A.adj_l = d_adj_lights;
+    A.adj_e = d_adj_env;
+    A.grad = d_grad_fog;
+    hipStream_t s = stream ? (hipStream_t)stream : k->stream;
+    return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, nullptr, nullptr, nullptr, nullptr, s,
+                       false, false, &A);
+}
+
+int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+                       uint64_t seed, uint32_t sample_base, const float *adj_lights, const float *adj_env,
+                       float *grad_fog) {
+    if (!k || !adj_lights || !adj_env || !grad_fog) return set_error(DRK_EINVAL, "drk_kernel_adjoint: NULL argument");
+    if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
+        return set_error(DRK_EINVAL, "drk_kernel_adjoint: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley,
+                         k->tilex, k->tiley);
+    const size_t tb = (size_t)tilex * tiley * 16;
+    const size_t gn = (size_t)k->scene->leafx[1] * k->P.lvoxc3;
+    float *d_adj = nullptr, *d_grad = nullptr;
+    hipError_t e = hipMalloc(&d_adj, 2 * tb);
+    if (e == hipSuccess) e = hipMalloc(&d_grad, gn * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_adj, adj_lights, tb, hipMemcpyHostToDevice, k->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(reinterpret_cast<char *>(d_adj) + tb, adj_env, tb, hipMemcpyHostToDevice, k->stream);
+    if (e == hipSuccess) e = launch_fill(d_grad, gn, 0.0f, k->stream);
+    int rc = DRK_OK;
+    if (e == hipSuccess) {
+        rc = drk_kernel_adjoint_device(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_adj,
+                                       reinterpret_cast<const float *>(reinterpret_cast<char *>(d_adj) + tb), d_grad,
+                                       k->stream);
+        if (rc == DRK_OK) e = hipMemcpyAsync(grad_fog, d_grad, gn * sizeof(float), hipMemcpyDeviceToHost, k->stream);
+        if (rc == DRK_OK && e == hipSuccess) e = hipStreamSynchronize(k->stream);
+    }
+    if (d_adj) (void)hipFree(d_adj);
+    if (d_grad) (void)hipFree(d_grad);
+    if (rc != DRK_OK) return rc;
+    if (e != hipSuccess) return hip_fail(e, "drk_kernel_adjoint");
+    return DRK_OK;
+}
+
 int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples) {
-    (void)x0; (void)y0; (void)tilex; (void)tiley; (void)samples;
     if (!k) return set_error(DRK_EINVAL, "drk_kernel_shadow: NULL kernel");
-    return set_error(DRK_EINVAL, "drk_kernel_shadow: the Shadow pass is not implemented yet");
+    size_t tb = (size_t)k->tilex * k->tiley * 16;
+    char *base = reinterpret_cast<char *>(k->d_out);
+    int rc = render_impl(k, x0, y0, tilex, tiley, samples, k->seed, k->sample_base, (float *)base,
+                         (float *)(base + tb), (float *)(base + 2 * tb), (float *)(base + 3 * tb), k->stream, false, true);
+    if (rc != DRK_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(k->h_phb, k->d_out, 4 * tb, hipMemcpyDeviceToHost, k->stream), "drk_kernel_shadow: copy-out");
+    HIP_TRY(hipStreamSynchronize(k->stream), "drk_kernel_shadow: sync");
+    k->sample_base += samples;
+    return DRK_OK;
 }
 
 int drk_kernel_set_timing(drk_kernel *k, int enable) {

@@ -68,6 +68,8 @@ DRK_HD uint32_t rng_key1(uint64_t seed) {
 // Counter purposes (counter word w).
 constexpr uint32_t kRngTrack = 0;    // free-flight / acceptance pairs of a tracked ray
 constexpr uint32_t kRngScatter = 1;  // phase + light direction samples at a scattering event
+constexpr uint32_t kRngShadowPass = 2;  // sun direction of a Shadow-pass sample
+constexpr uint32_t kRngAdjoint = 3;     // adjoint integral sample points of a replayed ray
 
 // Ray identifiers inside one path: the camera ray is 0; scattering event e
 // (at depth e) spawns the shadow ray 2e+1 and the continuation ray 2e+2.

@@ -36,8 +36,9 @@ __device__ __forceinline__ void load_mie(MieLds &m) {
 // ---------------------------------------------------------------------------
 struct RayRng {
     uint32_t s0, s1, s2, s3;
-    __device__ __forceinline__ void seed(uint32_t key0, uint32_t key1, uint32_t sample, uint32_t ray) {
-        U4 v = philox4x32_10(U4{sample, ray, 0u, kRngTrack}, key0, key1);
+    __device__ __forceinline__ void seed(uint32_t key0, uint32_t key1, uint32_t sample, uint32_t ray,
+                                         uint32_t purpose = kRngTrack) {
+        U4 v = philox4x32_10(U4{sample, ray, 0u, purpose}, key0, key1);
         s0 = v.x; s1 = v.y; s2 = v.z; s3 = v.w | ((v.x | v.y | v.z) == 0u ? 1u : 0u);
     }
     __device__ __forceinline__ uint32_t next() {
@@ -243,6 +244,37 @@ __device__ __forceinline__ float brick_lookup(const float *__restrict__ brick, F
     float v0 = nl[1] * (u10 - u00) + u00;
     float v1 = nl[1] * (u11 - u01) + u01;
     return nl[2] * (v1 - v0) + v0;
+}
+
+// Derivative of brick_lookup with respect to the 8 corner values it reads:
+// corner index (z*l^2 + y*l + x) and weight, with exactly brick_lookup's index
+// and fraction arithmetic (d lerp(a,b,t)/da = 1-t, /db = t); clamped corners
+// may repeat, their weights then add up.
+__device__ __forceinline__ void brick_weights(F3 p, float4 ce, uint32_t lvoxc, uint32_t idx[8], float w[8]) {
+    const float nv = (float)lvoxc;
+    const float nvm = nv - 1.0f;
+    float ni[3] = {((-0.5f * ((ce.x - ce.w) - p.x)) * nvm) / ce.w,
+                   ((-0.5f * ((ce.y - ce.w) - p.y)) * nvm) / ce.w,
+                   ((-0.5f * ((ce.z - ce.w) - p.z)) * nvm) / ce.w};
+    uint32_t ia[3], ib[3];
+    float nl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float nf = fmaxf(floorf(ni[k]), 0.0f);
+        float nc = fminf(ceilf(ni[k]), nvm);
+        nl[k] = ni[k] - nf;
+        nc = fmaxf(fminf(nc, nvm), 0.0f);
+        nf = fmaxf(fminf(nf, nvm), 0.0f);
+        ia[k] = (uint32_t)nf;
+        ib[k] = (uint32_t)nc;
+    }
+    const uint32_t l2 = lvoxc * lvoxc;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const bool bx = v & 1, by = v & 2, bz = v & 4;
+        idx[v] = (bz ? ib[2] : ia[2]) * l2 + (by ? ib[1] : ia[1]) * lvoxc + (bx ? ib[0] : ia[0]);
+        w[v] = (bx ? nl[0] : 1.0f - nl[0]) * (by ? nl[1] : 1.0f - nl[1]) * (bz ? nl[2] : 1.0f - nl[2]);
+    }
 }
 
 // ---------------------------------------------------------------------------

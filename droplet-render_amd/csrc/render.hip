@@ -12,6 +12,15 @@
 // (src/kernel.cpp:128-445) unrolled into a loop: each scattering event
 // spawns one shadow ray toward lights[0] and one continuation ray whose
 // throughput carries the balance-heuristic weight and the albedo.
+//
+// The same state machine, instantiated in adjoint mode, computes the
+// gradient of <dL/dimage, image> with respect to the FOG brick densities by
+// path replay: pass 0 re-traces the path to get its adjoint-weighted total
+// and which shadow rays escape; pass 1 re-traces it again and, for each
+// tracked segment with downstream adjoint radiance W, adds W * w_v(x)/rho(x)
+// at the real collision and -W * sigma_e * integral of w_v over the leaf
+// intervals the tracker covered (one uniform point per leaf, from a separate
+// replayable stream). The majorants are held fixed. No path is stored.
 #include <hip/hip_runtime.h>
 
 #include "drk_common.h"
@@ -23,7 +32,7 @@ namespace drk {
 constexpr int kBlock = 256;
 
 enum : uint32_t { ST_WORK = 0, ST_TRAV = 1, ST_LEAF = 2, ST_END = 3, ST_DONE = 4 };
-enum : uint32_t { RK_PRIMARY = 0, RK_CONT = 1, RK_SHADOW = 2 };
+enum : uint32_t { RK_PRIMARY = 0, RK_CONT = 1, RK_SHADOW = 2, RK_SHADOW_PASS = 3 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
@@ -39,8 +48,11 @@ __device__ __forceinline__ uint32_t fetch_units(uint32_t *counter, bool need) {
     return base + below;
 }
 
-template <bool kSumSq, bool kCount>
+template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W) {
+    constexpr bool kSumSq = (kMode == kModeForwardSq || kMode == kModeCount);
+    constexpr bool kCount = (kMode == kModeCount);
+    constexpr bool kAdj = (kMode == kModeAdjoint);
     __shared__ MieLds mie;
     __shared__ uint32_t stk_all[kMaxDepth * kBlock];
     load_mie(mie);
@@ -73,15 +85,29 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
     // Woodcock state of the current leaf.
     float4 ce = make_float4(0, 0, 0, 1);
     const float *brick = nullptr;
+    size_t brick_off = 0;
     float smax = 1.0f, kmaj = 1.0f, lo = 0.0f, s0 = 0.0f, tr0 = 0.0f, tr1 = 0.0f, trb = 0.0f, sr = 0.0f, td = 0.0f;
     bool first_step = true;
     uint32_t cnt_nodes = 0, cnt_lookups = 0;
+    // Adjoint state.
+    F3 gL = mk3(0, 0, 0), gE = mk3(0, 0, 0);
+    float gA = 0.0f, ltot = 0.0f, collected = 0.0f, wseg = 0.0f, seg_a = 0.0f;
+    uint32_t pass = 0;
+    uint64_t vis = 0;
+    RayRng arng;
+    arng.s0 = arng.s1 = arng.s2 = arng.s3 = 1u;
 
     auto start_ray = [&](uint32_t ray_id) {
-        rng.seed(key0, P.key1, P.sample_base + s_cur, ray_id);
         td = 0.0f;
         hit = false;
         T.lvl = -1;
+        if (kAdj && pass == 1 && kind == RK_SHADOW && !((vis >> event) & 1ull)) {
+            hit = true;   // blocked in pass 0: no contribution, no gradient, not re-traced
+            st = ST_END;
+            return;
+        }
+        rng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngTrack);
+        if (kAdj && pass == 1) arng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngAdjoint);
         F3 t0, t1;
         uint32_t a;
         if (!octree_init(P.ob, ro, rd, t0, t1, a) || t1.x < 0.0f || t1.y < 0.0f || t1.z < 0.0f) {
@@ -93,11 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
         if (is_leaf(P.ob[0])) {
             T.lvl = -2;    // root leaf pending
         } else {
-            F3 tm;
-            {
-#pragma clang fp contract(off)
-                tm = mk3(0.5f * (t0.x + t1.x), 0.5f * (t0.y + t1.y), 0.5f * (t0.z + t1.z));
-            }
+            F3 tm = mk3(0.5f * (t0.x + t1.x), 0.5f * (t0.y + t1.y), 0.5f * (t0.z + t1.z));
             stk[0] = 0u | (first_node(t0, tm) << 24);
             T.lvl = 0;
         }
@@ -113,8 +135,47 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
         alpha = 0.0f;
         depth_r = 0;
         kind = RK_PRIMARY;
+        if (kAdj && pass == 1) {
+            collected = 0.0f;
+            wseg = ltot;   // the camera segment carries the whole path
+        }
+        if (P.shadow_mode) {
+            // RenderKernel::Shadow (src/kernel.cpp:569-607): a sun-sampled
+            // shadow ray from the surface point at the compositing depth.
+            kind = RK_SHADOW_PASS;
+            const bool gm1 = gm && pix_depth < P.clip_end;
+            const F3 surf = mk3(ro.x + rd.x * pix_depth, ro.y + rd.y * pix_depth, ro.z + rd.z * pix_depth);
+            U4 r4 = philox4x32_10(U4{P.sample_base + s_cur, 0u, 0u, kRngShadowPass}, key0, P.key1);
+            rd = sun_sample(P, u01(r4.x), u01(r4.y));
+            if (!gm1) { st = ST_END; hit = false; return; }   // untraced lane: unshadowed
+            ro = surf;
+            start_ray(shadow_ray_id(0));
+            return;
+        }
         if (!gm) { st = ST_END; hit = false; return; }
         start_ray(0);
+    };
+
+    // Adds weight * (trilinear weight of vertex v at x) to every corner v of
+    // the current leaf's gradient brick (adjoint pass 1 only).
+    auto grad_at = [&](F3 x, float weight) {
+        if (weight == 0.0f) return;
+        uint32_t idx[8];
+        float w8[8];
+        brick_weights(x, ce, P.lvoxc, idx, w8);
+        float *g = W.grad + brick_off;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) unsafeAtomicAdd(g + idx[v], weight * w8[v]);
+    };
+    // -W * sigma_e * integral over [a, b] of the trilinear weights, by one
+    // uniform point of the adjoint stream (drawn for every covered leaf with a
+    // non-zero majorant, so the stream stays aligned with the oracle).
+    auto grad_segment = [&](float a, float b) {
+        float u = arng.uniform();
+        if (!(b > a)) return;
+        float t = a + u * (b - a);
+        F3 x = mk3(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
+        grad_at(x, -wseg * sig_e * (b - a));
     };
 
     for (;;) {
@@ -141,7 +202,21 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     accA = 0.0f;
                     sqL[0] = sqL[1] = sqL[2] = 0.0f;
                     sqE[0] = sqE[1] = sqE[2] = 0.0f;
-                    start_path();
+                    if (kAdj) {
+                        size_t d = (size_t)ly * P.tilex + lx;
+                        float4 a = reinterpret_cast<const float4 *>(W.adj_l)[d];
+                        float4 b = reinterpret_cast<const float4 *>(W.adj_e)[d];
+                        gL = mk3(a.x, a.y, a.z);
+                        gE = mk3(b.x, b.y, b.z);
+                        gA = a.w + b.w;   // alpha appears in both buffers
+                        pass = 0;
+                        vis = 0;
+                        if (gL.x == 0.0f && gL.y == 0.0f && gL.z == 0.0f && gE.x == 0.0f && gE.y == 0.0f &&
+                            gE.z == 0.0f && gA == 0.0f)
+                            s_cur = s_end;   // no adjoint on this pixel: nothing to differentiate
+                    }
+                    if (s_cur < s_end) start_path();
+                    else st = ST_WORK;
                 }
             }
         }
@@ -170,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 const drk_octree_node &n = P.ob[leaf];
                 ce = *reinterpret_cast<const float4 *>(n.ce);
                 smax = n.qval[DRK_VOLUME_FOG];
-                brick = P.pvol_fog + (size_t)n.volx[DRK_VOLUME_FOG] * P.lvoxc3;
+                brick_off = (size_t)n.volx[DRK_VOLUME_FOG] * P.lvoxc3;
+                brick = P.pvol_fog + brick_off;
                 if (kCount) cnt_nodes += 1;
                 // Fog-only leaf: always skip to the leaf entry (src/kernel.cpp:233-269).
                 lo = td;
@@ -178,6 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 tr1 = trbn - td;
                 trb = trbn;
                 td = tra;
+                seg_a = tra;
                 s0 = tr0;
                 kmaj = sig_e * smax;
                 first_step = true;
@@ -195,21 +272,22 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
             if (!(sc < tr1)) {
                 td = trb;     // left the leaf: memoryless restart in the next one
                 st = ST_TRAV;
+                if (kAdj && pass == 1 && kmaj > 0.0f) grad_segment(seg_a, trb);
             } else if (!(sc > tr0)) {
                 hit = true;   // collision before the leaf entry counts as a real one (src/kernel.cpp:287)
                 st = ST_END;
             } else {
-                F3 rc;
-                {
-#pragma clang fp contract(off)
-                    rc = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
-                }
+                F3 rc = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
                 float p = brick_lookup(brick, rc, ce, P.lvoxc);
                 if (kCount) cnt_lookups += 1;
                 float q = rng.uniform();
                 if (!(p / smax < q)) {
                     hit = true;
                     st = ST_END;
+                    if (kAdj && pass == 1) {
+                        grad_segment(seg_a, td);
+                        grad_at(rc, wseg / p);   // d sigma / sigma at the real collision
+                    }
                 }
             }
         }
@@ -217,18 +295,29 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
         // ---- C: ray finished (escaped or collided) -------------------------
         if (st == ST_END) {
             bool path_done = false;
-            if (kind == RK_SHADOW) {
+            if (kind == RK_SHADOW_PASS) {
+                // SampleVolume at r = scattevs returns the lights' radiance when
+                // the ray escapes (r > 0); Shadow divides by lights[0]'s colour.
+                if (!hit && P.scattevs > 0) {
+                    F3 le = lights_eval(P, rd);
+                    pathL = mk3(le.x / P.sun_color[0][0], le.y / P.sun_color[0][1], le.z / P.sun_color[0][2]);
+                }
+                alpha = 1.0f;
+                path_done = true;
+            } else if (kind == RK_SHADOW) {
                 if (!hit) {
                     F3 le = lights_eval(P, rd);
                     pathL.x += shadow_w.x * le.x;
                     pathL.y += shadow_w.y * le.y;
                     pathL.z += shadow_w.z * le.z;
+                    if (kAdj && pass == 0) vis |= 1ull << event;
                 }
                 // Continue the path along the phase-sampled direction.
                 rd = cont_dir;
                 beta = cont_beta;
                 depth_r += 1;
                 kind = RK_CONT;
+                if (kAdj && pass == 1) wseg = ltot - collected;
                 start_ray(cont_ray_id(event));
             } else if (!hit) {
                 if (depth_r > 0) {
@@ -239,7 +328,10 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 }
                 path_done = true;
             } else {
-                if (depth_r == 0) alpha = 1.0f;
+                if (depth_r == 0) {
+                    alpha = 1.0f;
+                    if (kAdj && pass == 1) collected += gA;
+                }
                 if (depth_r >= P.scattevs) {
                     path_done = true;
                 } else {
@@ -263,11 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     float pd1 = sun_pdf(P, srd), pd2 = sun_pdf(P, lrd);
                     F3 w1 = mk3(p1.x / (p1.x + pd1), p1.y / (p1.y + pd1), p1.z / (p1.z + pd1));
                     F3 w2 = mk3(p3.x / (p3.x + pd2), p3.y / (p3.y + pd2), p3.z / (p3.z + pd2));
-                    F3 x;
-                    {
-#pragma clang fp contract(off)
-                        x = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
-                    }
+                    F3 x = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
                     shadow_w = mk3(beta.x * w2.x * sig_s / sig_e,
                                    beta.y * w2.y * sig_s / sig_e,
                                    beta.z * w2.z * sig_s / sig_e);
@@ -278,10 +366,36 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     ro = x;
                     rd = lrd;
                     kind = RK_SHADOW;
+                    if (kAdj && pass == 1) {
+                        // The shadow ray's segment carries only its own contribution.
+                        wseg = 0.0f;
+                        if ((vis >> event) & 1ull) {
+                            F3 le = lights_eval(P, lrd);
+                            wseg = gL.x * (shadow_w.x * le.x) + gL.y * (shadow_w.y * le.y) + gL.z * (shadow_w.z * le.z);
+                            collected += wseg;
+                        }
+                    }
                     start_ray(shadow_ray_id(event));
                 }
             }
-            if (path_done) {
+            if (path_done && kAdj) {
+                if (pass == 0) {
+                    ltot = gL.x * pathL.x + gL.y * pathL.y + gL.z * pathL.z + gE.x * pathE.x + gE.y * pathE.y +
+                           gE.z * pathE.z + gA * alpha;
+                    // Replay the same path for the gradients. (A zero total does not
+                    // mean zero sub-path weights when adjoints have mixed signs.)
+                    pass = 1;
+                    start_path();
+                    path_done = false;
+                }
+                if (path_done) {
+                    pass = 0;
+                    vis = 0;
+                    ++s_cur;
+                    if (s_cur < s_end) start_path();
+                    else st = ST_WORK;
+                }
+            } else if (path_done) {
                 accL[0] += pathL.x; accL[1] += pathL.y; accL[2] += pathL.z;
                 accE[0] += pathE.x; accE[1] += pathE.y; accE[2] += pathE.z;
                 accA += alpha;
@@ -360,29 +474,36 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 // ---------------------------------------------------------------------------
 int render_block_size() { return kBlock; }
 
-int render_blocks_per_cu(bool sumsq, bool count) {
+int render_blocks_per_cu(int mode) {
     int nb = 0;
     hipError_t e;
-    if (count) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<true, true>, kBlock, 0);
-    else if (sumsq) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<true, false>, kBlock, 0);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<false, false>, kBlock, 0);
+    switch (mode) {
+    case kModeForward: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeForward>, kBlock, 0); break;
+    case kModeForwardSq: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeForwardSq>, kBlock, 0); break;
+    case kModeCount: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeCount>, kBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeAdjoint>, kBlock, 0); break;
+    }
     if (e != hipSuccess || nb <= 0) return 1;
     return nb;
 }
 
-hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, bool sumsq, bool count,
-                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
+                         hipEvent_t ev0, hipEvent_t ev1) {
     k_fill_u32<<<1, 64, 0, s>>>(W.counter, 1, 0u);
     if (ev0) (void)hipEventRecord(ev0, s);
-    if (count) k_render<true, true><<<grid, kBlock, 0, s>>>(P, W);
-    else if (sumsq) k_render<true, false><<<grid, kBlock, 0, s>>>(P, W);
-    else k_render<false, false><<<grid, kBlock, 0, s>>>(P, W);
+    switch (mode) {
+    case kModeForward: k_render<kModeForward><<<grid, kBlock, 0, s>>>(P, W); break;
+    case kModeForwardSq: k_render<kModeForwardSq><<<grid, kBlock, 0, s>>>(P, W); break;
+    case kModeCount: k_render<kModeCount><<<grid, kBlock, 0, s>>>(P, W); break;
+    default: k_render<kModeAdjoint><<<grid, kBlock, 0, s>>>(P, W); break;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev1) (void)hipEventRecord(ev1, s);
-    if (W.partial) {
+    if (W.partial && mode != kModeAdjoint) {
         uint32_t nb = (W.npix + 255) / 256;
-        k_reduce<<<nb, 256, 0, s>>>(W, P.rx, P.tilex, sumsq ? 4u : 2u);
+        bool sq = (mode == kModeForwardSq || mode == kModeCount);
+        k_reduce<<<nb, 256, 0, s>>>(W, P.rx, P.tilex, sq ? 4u : 2u);
         e = hipGetLastError();
     }
     return e;

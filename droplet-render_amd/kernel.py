@@ -69,9 +69,11 @@ class RenderKernel:
         self.tilew, self.tileh = int(tilex), int(tiley)
 
     def _buf(self, b):
+        # Packed with the stride of the last call's tile width, like the
+        # reference's phb (src/kernel.cpp:560-564).
         p = L.lib().drk_kernel_phb(self._h, b)
-        a = np.ctypeslib.as_array(p, shape=(self.tiley * self.tilex * 4,))
-        return a.reshape(self.tiley, self.tilex, 4)[: self.tileh, : self.tilew].copy()
+        n = self.tileh * self.tilew * 4
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.tileh, self.tilew, 4).copy()
 
     @property
     def phb(self):
@@ -95,6 +97,26 @@ class RenderKernel:
         L.check(L.lib().drk_kernel_render_device(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples),
                                                  int(seed), int(sample_base), ptr(out_l), ptr(out_e), ptr(sq_l),
                                                  ptr(sq_e), s))
+
+    def adjoint(self, x0, y0, tilex, tiley, samples, seed, sample_base, adj_lights, adj_env):
+        """dL/d(FOG brick values) for the render with these arguments, given
+        dL/d(phb sums) of both buffers as (tiley, tilex, 4) arrays (host)."""
+        al = np.ascontiguousarray(adj_lights, dtype=np.float32).reshape(-1)
+        ae = np.ascontiguousarray(adj_env, dtype=np.float32).reshape(-1)
+        assert al.size == tilex * tiley * 4 and ae.size == al.size
+        grad = np.zeros(self.scene.leafx[1] * self.scene.lvoxc3, np.float32)
+        L.check(L.lib().drk_kernel_adjoint(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples),
+                                           int(seed), int(sample_base), L.fptr(al), L.fptr(ae), L.fptr(grad)))
+        return grad
+
+    def adjoint_device(self, x0, y0, tilex, tiley, samples, seed, sample_base, adj_l, adj_e, grad, stream=None):
+        """Device form: torch tensors; accumulates into grad (asynchronous)."""
+        for t in (adj_l, adj_e, grad):
+            assert t.is_cuda and t.dtype.is_floating_point and t.is_contiguous()
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_kernel_adjoint_device(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples),
+                                                  int(seed), int(sample_base), C.c_void_p(adj_l.data_ptr()),
+                                                  C.c_void_p(adj_e.data_ptr()), C.c_void_p(grad.data_ptr()), s))
 
     def count(self, x0, y0, tilex, tiley, samples, seed, sample_base=0):
         """Node-record reads and brick lookups of one render (counting pass)."""

@@ -82,6 +82,20 @@ class Scene:
         if q is not None:
             self.ob["qval"][:, L.DRK_VOLUME_FOG] = q
 
+    def raise_majorants(self, factor=1.25, floor=None):
+        """Multiplies every non-zero FOG majorant by `factor` (any value at or
+        above the brick maximum keeps delta tracking unbiased); used to keep
+        density perturbations and optimisation steps under fixed majorants.
+        Leaves with majorant 0 (never resampled) stay transparent."""
+        q = self.ob["qval"][:, L.DRK_VOLUME_FOG].copy()
+        m = q > 0
+        q[m] *= np.float32(factor)
+        if floor is not None:
+            q[m] = np.maximum(q[m], np.float32(floor))
+        self.ob["qval"][:, L.DRK_VOLUME_FOG] = q
+        if self._dev is not None:
+            L.check(L.lib().drk_scene_update_fog(self._dev, L.fptr(self.pvol[1]), L.fptr(q)))
+
     def destroy(self):
         if self._dev is not None:
             L.lib().drk_scene_destroy(self._dev)

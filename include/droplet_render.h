@@ -167,6 +167,26 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
                              void *stream);
 
 /*
+ * Adjoint of Render (no reference counterpart; the reference is not
+ * differentiable). Given dL/d(phb sums) for the lights and environment
+ * buffers (tilex*tiley RGBA floats each; the alpha gradient counts once per
+ * buffer), accumulates dL/d(FOG brick values) into grad_fog (leafx*lvoxc^3
+ * floats, the layout of Scene::pvol[FOG]) for the render with the same tile,
+ * samples, seed and sample_base. The majorants (qval) are held fixed. The
+ * estimator is path replay with differential delta tracking: unbiased for the
+ * derivative of the expected image, no stored paths.
+ * _device: device pointers, accumulates (the caller zeroes grad), async on
+ * stream. Host form: host pointers, grad_fog overwritten, synchronous.
+ * Not supported with depth compositing or scattevs > 64.
+ */
+int drk_kernel_adjoint_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                              uint32_t samples, uint64_t seed, uint32_t sample_base, const float *d_adj_lights,
+                              const float *d_adj_env, float *d_grad_fog, void *stream);
+int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+                       uint64_t seed, uint32_t sample_base, const float *adj_lights, const float *adj_env,
+                       float *grad_fog);
+
+/*
  * Extensions with no reference counterpart.
  *
  * drk_kernel_set_tuning: fixes the work-unit chunk (samples per unit) and the

@@ -726,5 +726,283 @@ int or_render(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32
     return 0;
 }
 
+/* ------------------------------------------------------------- adjoint -- */
+/*
+ * Derivative of the expected image with respect to the FOG brick values,
+ * majorants fixed (no reference counterpart: the reference is not
+ * differentiable). Differential delta tracking, written over the reference's
+ * recursion: a tracked ray whose subtree has adjoint-weighted value W adds
+ *   W * w_v(x)/rho(x)                      at its real collision x,
+ *   -W * sigma_e * (b-a) * w_v(a+u(b-a))   per covered leaf interval [a,b]
+ *                                          with a non-zero majorant,
+ * w_v the trilinear weights of vertex v; u comes from the ray's adjoint
+ * stream (Philox counter purpose 3), one draw per covered leaf.
+ */
+typedef struct {
+    float *grad;             /* leafx * lvoxc^3 */
+    float gL[3], gE[3], gA;
+} oadj;
+
+/* Corner indices and weights of or_voxel's lerps (d lerp/da = 1-t, /db = t). */
+static void voxel_weights(const float p[3], const float ce[4], uint32_t lvoxc, uint32_t idx[8], float w[8]) {
+    float nv = (float)lvoxc, nvm = nv - 1.0f, nl[3];
+    uint32_t a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        float ni = ((-0.5f * ((ce[k] - ce[3]) - p[k])) * nvm) / ce[3];
+        float nf = fmaxf(floorf(ni), 0.0f);
+        float nc = fminf(ceilf(ni), nvm);
+        nl[k] = ni - nf;
+        nc = fmaxf(fminf(nc, (float)(lvoxc - 1)), 0.0f);
+        nf = fmaxf(fminf(nf, (float)(lvoxc - 1)), 0.0f);
+        a[k] = (uint32_t)nf;
+        b[k] = (uint32_t)nc;
+    }
+    for (int v = 0; v < 8; ++v) {
+        int bx = v & 1, by = (v >> 1) & 1, bz = (v >> 2) & 1;
+        idx[v] = (bz ? b[2] : a[2]) * lvoxc * lvoxc + (by ? b[1] : a[1]) * lvoxc + (bx ? b[0] : a[0]);
+        w[v] = (bx ? nl[0] : 1.0f - nl[0]) * (by ? nl[1] : 1.0f - nl[1]) * (bz ? nl[2] : 1.0f - nl[2]);
+    }
+}
+
+static void add_grad(const oparams *P, oadj *A, const onode *N, const float x[3], float weight) {
+    if (weight == 0.0f) return;
+    uint32_t idx[8];
+    float w[8];
+    voxel_weights(x, N->ce, P->lvoxc, idx, w);
+    float *g = A->grad + (size_t)N->volx[1] * P->lvoxc3;
+    for (int v = 0; v < 8; ++v) g[idx[v]] += weight * w[v];
+}
+
+static void add_segment(const oparams *P, oadj *A, const onode *N, ostream *ar, const float ro[3], const float rd[3],
+                        float a, float b, float W) {
+    float u = stream_u(ar);
+    if (!(b > a)) return;
+    float t = a + u * (b - a);
+    float x[3] = {ro[0] + rd[0] * t, ro[1] + rd[1] * t, ro[2] + rd[2] * t};
+    add_grad(P, A, N, x, -W * (P->sigma_a + P->sigma_s) * (b - a));
+}
+
+/* Replays track() for ray `ray_id` and adds its derivative terms with weight W. */
+static void track_adj(octx *X, oadj *A, const float ro[3], const float rd[3], uint32_t ray_id, float depth, float W) {
+    const oparams *P = X->P;
+    ostream rng, ar;
+    stream_seed(&rng, X->key0, P->key1, X->sample, ray_id);
+    {
+        uint32_t c[4] = {X->sample, ray_id, 0u, 3u}, v[4];
+        philox(c, X->key0, P->key1, v);
+        ar.s[0] = v[0]; ar.s[1] = v[1]; ar.s[2] = v[2];
+        ar.s[3] = v[3] | ((v[0] | v[1] | v[2]) == 0u ? 1u : 0u);
+    }
+    traverse(&X->L, ro, rd);
+    const float sige = P->sigma_a + P->sigma_s;
+    float td = 0.0f;
+    for (uint32_t i = 0; i < X->L.n; ++i) {
+        if (!(td < depth)) break;
+        const oleaf *lf = &X->L.ls[i];
+        const onode *N = &P->ob[lf->node];
+        float tra = lf->t0, trb = lf->t1, lo = td, tr0 = tra - td, tr1 = trb - td, smax = N->qval[1];
+        td = tra;
+        const float *brick = P->pvol_fog + (size_t)N->volx[1] * P->lvoxc3;
+        float sr = -(logf(stream_u(&rng)) / (sige * smax));
+        for (;;) {
+            float sc = tr0 + sr;
+            td = lo + sc;
+            if (!(sc < tr1)) {
+                td = trb;
+                if (sige * smax > 0.0f) add_segment(P, A, N, &ar, ro, rd, tra, trb, W);
+                break;
+            }
+            if (!(sc > tr0)) return;          /* collision before the leaf entry: no density term */
+            float rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
+            float p = or_voxel(brick, rc, N->ce, P->lvoxc);
+            float q = stream_u(&rng);
+            if (!(p / smax < q)) {
+                add_segment(P, A, N, &ar, ro, rd, tra, td, W);
+                add_grad(P, A, N, rc, W / p);
+                return;
+            }
+            sr = sr - logf(stream_u(&rng)) / (sige * smax);
+        }
+    }
+}
+
+/* sample_volume with the adjoint: same value, plus this ray's and its
+ * children's derivative terms. mL/mE: per-channel multipliers from the camera
+ * sample down to this call. */
+static void sample_volume_adj(octx *X, oadj *A, const float ro[3], const float rd[3], uint32_t r, uint32_t ray_id,
+                              float depth, const float mL[3], const float mE[3], ores *out) {
+    const oparams *P = X->P;
+    memset(out, 0, sizeof(*out));
+    float td;
+    int scattered = track(X, ro, rd, ray_id, depth, &td);
+    int rm = !scattered;
+    float lc[4] = {0, 0, 0, 0}, le[4] = {0, 0, 0, 0};
+    if (rm && r > 0) {
+        for (uint32_t l = 0; l < P->light_count; ++l)
+            if (dot3(rd, P->lights[l].dir) > P->lights[l].cosang)
+                for (int k = 0; k < 3; ++k) lc[k] += P->lights[l].color[k];
+        env_eval(P, rd, le);
+        lc[3] = le[3] = 1.0f;
+    } else {
+        lc[3] = le[3] = rm ? 0.0f : 1.0f;
+    }
+    if (r < P->scattevs && !rm) {
+        uint32_t c[4] = {X->sample, r, 0u, 1u}, v[4];
+        philox(c, X->key0, P->key1, v);
+        float u1 = u01(v[0]), u2 = u01(v[1]), u3 = u01(v[2]), u4 = u01(v[3]);
+        float srd[3], lrd[3], p1[3], p3[3];
+        if (P->phase == 0) or_mie_sample(P->mie, rd, u1, u2, srd);
+        else or_hg_sample(P->hg_g, rd, u1, u2, srd);
+        sun_sample(&P->lights[0], u3, u4, lrd);
+        if (P->phase == 0) {
+            or_mie_eval(P->mie, dot3(srd, rd), p1);
+            or_mie_eval(P->mie, dot3(lrd, rd), p3);
+        } else {
+            float h1 = or_hg_eval(P->hg_g, dot3(srd, rd)), h3 = or_hg_eval(P->hg_g, dot3(lrd, rd));
+            p1[0] = p1[1] = p1[2] = h1;
+            p3[0] = p3[1] = p3[2] = h3;
+        }
+        float p2 = sun_pdf(&P->lights[0], lrd), pd1 = sun_pdf(&P->lights[0], srd);
+        float sige = P->sigma_a + P->sigma_s;
+        float w1[3], w2[3], m1L[3], m1E[3], m2L[3], m2E[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+            w1[k] = p1[k] / (p1[k] + pd1);
+            w2[k] = p3[k] / (p3[k] + p2);
+            m1L[k] = mL[k] * w1[k] * P->sigma_s / sige;
+            m1E[k] = mE[k] * w1[k] * P->sigma_s / sige;
+            m2L[k] = mL[k] * w2[k] * P->sigma_s / sige;
+        }
+        float rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
+        ores S1, S2;
+        sample_volume_adj(X, A, rc, srd, r + 1, 2u * r + 2u, FLT_MAX, m1L, m1E, &S1);
+        sample_volume_adj(X, A, rc, lrd, P->scattevs, 2u * r + 1u, FLT_MAX, m2L, m2E, &S2);
+        for (int k = 0; k < 3; ++k) {
+            out->cl[k] += ((S1.cl[k] * w1[k] + S2.cl[k] * w2[k]) * P->sigma_s) / sige;
+            out->cs[k] += ((S1.cs[k] * w1[k] + 0.0f * w2[k]) * P->sigma_s) / sige;
+        }
+        out->cl[3] += lc[3];
+        out->cs[3] += le[3];
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            out->cl[k] += rm ? lc[k] : 0.0f;
+            out->cs[k] += rm ? le[k] : 0.0f;
+        }
+        out->cl[3] += lc[3];
+        out->cs[3] += le[3];
+    }
+    /* This ray's weight: the adjoint-weighted value of everything it leads to
+     * (its escape radiance, or the whole subtree of its collision), as seen
+     * from the camera sample. Its children added their own terms above. */
+    /* Channels with a zero multiplier never reach the image (a shadow ray's
+     * sky is dropped, sky2 = 0, and may be NaN near the sun): skip them. */
+    float Wray = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        if (A->gL[k] != 0.0f && mL[k] != 0.0f) Wray += A->gL[k] * mL[k] * out->cl[k];
+        if (A->gE[k] != 0.0f && mE[k] != 0.0f) Wray += A->gE[k] * mE[k] * out->cs[k];
+    }
+    if (r == 0) Wray += A->gA * out->cl[3];
+    if (Wray != 0.0f) track_adj(X, A, ro, rd, ray_id, depth, Wray);
+}
+
+/*
+ * Adjoint of or_render for a tile: adj holds dL/d(lights sums) then
+ * dL/d(env sums), each tiley*tilex*4 floats; grad (leafx*lvoxc^3) is
+ * overwritten with dL/d(FOG brick values).
+ */
+int or_adjoint(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+               uint32_t sample_base, const float *adj, float *grad, size_t grad_n) {
+    memset(grad, 0, sizeof(float) * grad_n);
+    size_t plane = (size_t)tilex * tiley * 4;
+    uint32_t rx = tilex & ~1u, ry = tiley & ~1u;
+    if (x0 + rx > P->w) rx = x0 < P->w ? P->w - x0 : 0;
+    if (y0 + ry > P->h) ry = y0 < P->h ? P->h - y0 : 0;
+    octx X;
+    memset(&X, 0, sizeof(X));
+    X.P = P;
+    X.L.ob = P->ob;
+    oadj A;
+    A.grad = grad;
+    const float one[3] = {1.0f, 1.0f, 1.0f};
+    for (uint32_t ly = 0; ly < ry; ++ly)
+        for (uint32_t lx = 0; lx < rx; ++lx) {
+            size_t o = ((size_t)ly * tilex + lx) * 4;
+            for (int k = 0; k < 3; ++k) {
+                A.gL[k] = adj[o + k];
+                A.gE[k] = adj[plane + o + k];
+            }
+            A.gA = adj[o + 3] + adj[plane + o + 3];
+            if (A.gL[0] == 0 && A.gL[1] == 0 && A.gL[2] == 0 && A.gE[0] == 0 && A.gE[1] == 0 && A.gE[2] == 0 && A.gA == 0)
+                continue;
+            uint32_t px = x0 + lx, py = y0 + ly;
+            float ro[3], rd[3];
+            if (!camera_ray(P, px, py, ro, rd)) continue;
+            X.key0 = py * P->w + px;
+            for (uint32_t s = 0; s < samples; ++s) {
+                X.sample = sample_base + s;
+                ores R;
+                sample_volume_adj(&X, &A, ro, rd, 0, 0u, FLT_MAX, one, one, &R);
+            }
+        }
+    free(X.L.ls);
+    return 0;
+}
+
+/*
+ * RenderKernel::Shadow (kernel.cpp:569-607): per pixel, the surface point at
+ * the compositing depth (when depth < clip_end) sends `samples` sun-sampled
+ * shadow rays, SampleVolume at r = scattevs; the lights' radiance of each
+ * escaping ray is divided by lights[0]'s colour and summed into out buffer 0
+ * (alpha: the sample count; the reference's alpha is 1/0 from a zero w). Lanes
+ * without a surface are not traced and count as unshadowed.
+ */
+int or_shadow(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+              uint32_t sample_base, float clip_end, float *out) {
+    size_t plane = (size_t)tilex * tiley * 4;
+    memset(out, 0, sizeof(float) * 4 * plane);
+    uint32_t rx = tilex & ~1u, ry = tiley & ~1u;
+    if (x0 + rx > P->w) rx = x0 < P->w ? P->w - x0 : 0;
+    if (y0 + ry > P->h) ry = y0 < P->h ? P->h - y0 : 0;
+    octx X;
+    memset(&X, 0, sizeof(X));
+    X.P = P;
+    X.L.ob = P->ob;
+    const olight *L0 = &P->lights[0];
+    for (uint32_t ly = 0; ly < ry; ++ly)
+        for (uint32_t lx = 0; lx < rx; ++lx) {
+            uint32_t px = x0 + lx, py = y0 + ly;
+            float ro[3], rd[3];
+            int gm = camera_ray(P, px, py, ro, rd);
+            if (!gm) continue;
+            X.key0 = py * P->w + px;
+            float depth = P->depth ? P->depth[X.key0] : FLT_MAX;
+            int gm1 = gm && depth < clip_end;
+            float ro1[3] = {ro[0] + rd[0] * depth, ro[1] + rd[1] * depth, ro[2] + rd[2] * depth};
+            double acc[4] = {0, 0, 0, 0};
+            for (uint32_t s = 0; s < samples; ++s) {
+                X.sample = sample_base + s;
+                uint32_t c[4] = {X.sample, 0u, 0u, 2u}, v[4];
+                philox(c, X.key0, P->key1, v);
+                float lrd[3];
+                sun_sample(L0, u01(v[0]), u01(v[1]), lrd);
+                float cl[3] = {0, 0, 0};
+                if (gm1) {
+                    ores R;
+                    sample_volume(&X, ro1, lrd, P->scattevs, 1u, FLT_MAX, &R);
+                    for (int k = 0; k < 3; ++k) cl[k] = R.cl[k];
+                } else if (P->scattevs > 0) {
+                    for (uint32_t l = 0; l < P->light_count; ++l)
+                        if (dot3(lrd, P->lights[l].dir) > P->lights[l].cosang)
+                            for (int k = 0; k < 3; ++k) cl[k] += P->lights[l].color[k];
+                }
+                for (int k = 0; k < 3; ++k) acc[k] += cl[k] / L0->color[k];
+                acc[3] += 1.0;
+            }
+            size_t o = ((size_t)ly * tilex + lx) * 4;
+            for (int k = 0; k < 4; ++k) out[o + k] = (float)acc[k];
+        }
+    free(X.L.ls);
+    return 0;
+}
+
 uint32_t or_params_size(void) { return (uint32_t)sizeof(oparams); }
 uint32_t or_mie_size(void) { return (uint32_t)sizeof(omie); }

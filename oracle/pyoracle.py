@@ -265,6 +265,43 @@ def render(wl, seed, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=
     return [out[0], out[1], out[2], out[3]]
 
 
+def adjoint(wl, seed, adj_lights, adj_env, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None):
+    """CPU adjoint: dL/d(FOG brick values) given dL/d(phb sums) of both buffers."""
+    tilex = tilex or wl.w
+    tiley = tiley or wl.h
+    samples = samples or wl.spp
+    P, keep = make_params(wl, seed)
+    adj = np.ascontiguousarray(np.stack([f32(adj_lights).reshape(tiley, tilex, 4),
+                                         f32(adj_env).reshape(tiley, tilex, 4)]))
+    n = wl.scene.leafx[1] * wl.scene.lvoxc3
+    grad = np.zeros(n, np.float32)
+    lib().or_adjoint(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
+                     C.c_uint32(samples), C.c_uint32(sample_base), fp(adj), fp(grad), C.c_size_t(n))
+    del keep
+    return grad
+
+
+def clip_end(proji):
+    """-z/w of (0,0,1,1)*proji in float32 (src/kernel.cpp:574-575)."""
+    p = f32(proji).reshape(4, 4)
+    z = np.float32(np.float32(np.float32(0.0) * p[0, 2] + np.float32(0.0) * p[1, 2]) + p[2, 2]) + p[3, 2]
+    w = np.float32(np.float32(np.float32(0.0) * p[0, 3] + np.float32(0.0) * p[1, 3]) + p[2, 3]) + p[3, 3]
+    return float(np.float32(-z / w))
+
+
+def shadow(wl, seed, depth, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None):
+    """CPU Shadow pass of a tile: (tiley, tilex, 4) sums (RGB visibility / sun colour, alpha = sample count)."""
+    tilex = tilex or wl.w
+    tiley = tiley or wl.h
+    samples = samples or wl.spp
+    P, keep = make_params(wl, seed, depth)
+    out = np.zeros((4, tiley, tilex, 4), np.float32)
+    lib().or_shadow(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
+                    C.c_uint32(samples), C.c_uint32(sample_base), C.c_float(clip_end(wl.proji)), fp(out))
+    del keep
+    return out[0]
+
+
 # ------------------------------------------------------- octree build (Python) --
 def _f(x):
     return np.float32(x)

@@ -441,7 +441,7 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t t
     if (!k || !d_lights || !d_env) return set_error(DRK_EINVAL, "drk_kernel_render_device: NULL argument");
     if ((d_sumsq_lights == nullptr) != (d_sumsq_env == nullptr))
         return set_error(DRK_EINVAL, "drk_kernel_render_device: pass both sum-of-squares buffers or neither");
-    hipStream_t s = stream ? (hipStream_t)stream : k->stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
     return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, d_sumsq_lights,
                        d_sumsq_env, s, false);
 }
@@ -488,7 +488,7 @@ int drk_kernel_adjoint_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t 
     A.adj_l = d_adj_lights;
     A.adj_e = d_adj_env;
     A.grad = d_grad_fog;
-    hipStream_t s = stream ? (hipStream_t)stream : k->stream;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
     return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, nullptr, nullptr, nullptr, nullptr, s,
                        false, false, &A);
 }

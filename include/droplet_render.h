@@ -156,8 +156,8 @@ void drk_kernel_destroy(drk_kernel *k);
  * Device-buffer form of Render for callers that keep results in HBM
  * (benchmarks, the autograd wrapper, batched drivers). Output pointers are
  * device pointers of tilex*tiley float4 each; the sums are written (not
- * accumulated). sumsq pointers may be NULL. stream is a hipStream_t (NULL =
- * the kernel's own stream). Asynchronous on that stream.
+ * accumulated). sumsq pointers may be NULL. stream is a hipStream_t (NULL = the
+ * default null stream, as in HIP). Asynchronous on that stream.
  */
 int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
                              uint32_t tilex, uint32_t tiley, uint32_t samples,

@@ -33,6 +33,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-steps", type=int, default=3)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -92,6 +94,35 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # Forward + adjoint: one step = the forward render plus the path-replay
+    # adjoint of the same samples for a dense image gradient (every pixel
+    # weighted), accumulated into FP32 gradient bricks.
+    fwd_adj = None
+    if not args.no_adjoint:
+        g = torch.Generator(device="cpu").manual_seed(5)
+        adj_l = (torch.rand(npix * 4, generator=g) * 1000.0).cuda()
+        adj_e = torch.rand(npix * 4, generator=g).cuda()
+        grad = torch.zeros(wl.scene.leafx[1] * wl.scene.lvoxc3, dtype=torch.float32, device="cuda")
+
+        def step_adj(i):
+            k.render_device(0, 0, wl.w, wl.h, wl.spp, args.seed, i * wl.spp, out_l, out_e, stream=stream)
+            k.adjoint_device(0, 0, wl.w, wl.h, wl.spp, args.seed, i * wl.spp, adj_l, adj_e, grad, stream=stream)
+
+        step_adj(0)
+        torch.cuda.synchronize()
+        k.timing()
+        ta = time.perf_counter()
+        for i in range(args.adj_steps):
+            step_adj(args.warmup + i)
+        torch.cuda.synchronize()
+        ta = time.perf_counter() - ta
+        kms, kn = k.timing()        # forward and adjoint launches alternate
+        fwd_adj = {"value": round(wl.samples * args.adj_steps / ta / 1e6, 3), "unit": "Msamples/s",
+                   "steps": args.adj_steps, "ms_per_step": round(ta / args.adj_steps * 1e3, 3),
+                   "ratio_to_forward": round((ta / args.adj_steps) / (wall / args.steps), 3),
+                   "adjoint_kernel_ms": round((kms - kern_avg_ms * kn / 2) / max(kn / 2, 1), 3),
+                   "gradient_finite": bool(torch.isfinite(grad).all().item())}
 
     cpu = None
     parity = None
@@ -158,6 +189,7 @@ def main():
             "alg_bytes_per_launch": int(alg_bytes), "node_reads_per_launch": int(n_node),
             "brick_lookups_per_launch": int(n_look),
         },
+        "fwd_adjoint": fwd_adj,
         "cpu_baseline": cpu,
         "parity": parity,
     }

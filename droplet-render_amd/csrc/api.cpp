@@ -68,6 +68,8 @@ struct drk_kernel {
     uint32_t *d_counter = nullptr;
     unsigned long long *d_counts = nullptr;
     float *d_env = nullptr;
+    KernelConsts C;                 // host copy of the per-kernel constants
+    KernelConsts *d_kc = nullptr;   // device copy read by the kernels
     float *d_depth = nullptr;
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
@@ -232,8 +234,6 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     P.pvol_sdf = scene->d_sdf;
     P.lvoxc = scene->lvoxc;
     P.lvoxc3 = scene->lvoxc * scene->lvoxc * scene->lvoxc;
-    std::memcpy(P.viewi, viewi, sizeof(P.viewi));
-    std::memcpy(P.proji, proji, sizeof(P.proji));
     P.w = w;
     P.h = h;
     P.scattevs = scattevs;
@@ -242,32 +242,42 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     P.phase = smp->phase;
     P.hg_g = smp->hg_g;
     P.env = smp->env;
-    P.light_count = smp->light_count;
+    KernelConsts &C = k->C;
+    std::memset(&C, 0, sizeof(C));
+    std::memcpy(C.viewi, viewi, sizeof(C.viewi));
+    std::memcpy(C.proji, proji, sizeof(C.proji));
+    C.light_count = smp->light_count;
     for (uint32_t l = 0; l < smp->light_count; ++l) {
         for (int c = 0; c < 3; ++c) {
-            P.sun_dir[l][c] = smp->lights[l].direction[c];
-            P.sun_color[l][c] = smp->lights[l].color[c];
+            C.sun_dir[l][c] = smp->lights[l].direction[c];
+            C.sun_color[l][c] = smp->lights[l].color[c];
         }
-        P.sun_cos[l] = cosf(smp->lights[l].angle);   // SunLight ctor, KernelSampler.cpp:306
+        C.sun_cos[l] = cosf(smp->lights[l].angle);   // SunLight ctor, KernelSampler.cpp:306
     }
     {
 #pragma clang fp contract(off)
-        float ctm = P.sun_cos[0];
-        P.sun_pdf0 = 1.0f / (2.0f * kPi * (1.0f - ctm * ctm));
+        float ctm = C.sun_cos[0];
+        C.sun_pdf0 = 1.0f / (2.0f * kPi * (1.0f - ctm * ctm));
     }
-    sampling_basis_host(P.sun_dir[0], P.sun_b1, P.sun_b2);
+    sampling_basis_host(C.sun_dir[0], C.sun_b1, C.sun_b2);
     if (smp->env == DRK_ENV_HOSEK) {
         // RenderKernel::Initialize, src/kernel.cpp:521-526, with the turbidity
         // and albedo made parameters.
-        float th = acosf(P.sun_dir[0][2]);
+        float th = acosf(C.sun_dir[0][2]);
         float se = 1.57079632679489661923f - th;
         double cfg[3][9], rad[3];
         hosek_state((double)smp->sky_turbidity, (double)smp->sky_albedo, (double)se, cfg, rad);
         for (int i = 0; i < 3; ++i) {
-            for (int j = 0; j < 9; ++j) P.sky.cfg[i][j] = (float)cfg[i][j];
-            P.sky.rad[i] = (float)rad[i];
+            for (int j = 0; j < 9; ++j) C.sky.cfg[i][j] = (float)cfg[i][j];
+            C.sky.rad[i] = (float)rad[i];
         }
-        for (int c = 0; c < 3; ++c) P.sky.dir[c] = P.sun_dir[0][c];
+        for (int c = 0; c < 3; ++c) C.sky.dir[c] = C.sun_dir[0][c];
+    }
+    {
+        // Far plane: -z/w of (0,0,1,1)*proji (src/kernel.cpp:574-575).
+        float zf_z = 0.0f * proji[0 * 4 + 2] + 0.0f * proji[1 * 4 + 2] + 1.0f * proji[2 * 4 + 2] + 1.0f * proji[3 * 4 + 2];
+        float zf_w = 0.0f * proji[0 * 4 + 3] + 0.0f * proji[1 * 4 + 3] + 1.0f * proji[2 * 4 + 3] + 1.0f * proji[3 * 4 + 3];
+        C.clip_end = -zf_z / zf_w;
     }
 
     hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
@@ -279,18 +289,15 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
         size_t eb = (size_t)smp->env_w * smp->env_h * 16;
         e = hipMalloc(&k->d_env, eb);
         if (e == hipSuccess) e = hipMemcpy(k->d_env, smp->env_map, eb, hipMemcpyHostToDevice);
-        P.env_map = k->d_env;
-        P.env_w = smp->env_w;
-        P.env_h = smp->env_h;
+        C.env_map = k->d_env;
+        C.env_w = smp->env_w;
+        C.env_h = smp->env_h;
     }
+    if (e == hipSuccess) e = hipMalloc(&k->d_kc, sizeof(KernelConsts));
+    if (e == hipSuccess) e = hipMemcpy(k->d_kc, &C, sizeof(KernelConsts), hipMemcpyHostToDevice);
+    P.kc = k->d_kc;
     // The depth map serves compositing (flag) and the Shadow pass (always).
     k->depthcomp = (flags & DRK_KERNEL_DEPTHCOMP) != 0;
-    {
-        // Far plane: -z/w of (0,0,1,1)*proji (src/kernel.cpp:574-575).
-        float zf_z = 0.0f * proji[0 * 4 + 2] + 0.0f * proji[1 * 4 + 2] + 1.0f * proji[2 * 4 + 2] + 1.0f * proji[3 * 4 + 2];
-        float zf_w = 0.0f * proji[0 * 4 + 3] + 0.0f * proji[1 * 4 + 3] + 1.0f * proji[2 * 4 + 3] + 1.0f * proji[3 * 4 + 3];
-        P.clip_end = -zf_z / zf_w;
-    }
     if (e == hipSuccess && pdepth) {
         e = hipMalloc(&k->d_depth, (size_t)w * h * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(k->d_depth, pdepth, (size_t)w * h * sizeof(float), hipMemcpyHostToDevice);
@@ -574,6 +581,7 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_counter) (void)hipFree(k->d_counter);
     if (k->d_counts) (void)hipFree(k->d_counts);
     if (k->d_env) (void)hipFree(k->d_env);
+    if (k->d_kc) (void)hipFree(k->d_kc);
     if (k->d_depth) (void)hipFree(k->d_depth);
     for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
     for (auto &pr : k->ev_free) {

@@ -85,14 +85,30 @@ struct SkyState {
     float dir[3];      // skydir = lights[0]->direction (src/kernel.cpp:522)
 };
 
+// Per-kernel constants (camera, lights, sky, environment): device-resident,
+// read through a pointer so the kernels load them on use instead of holding
+// them in scalar registers for the whole launch.
+struct KernelConsts {
+    float viewi[16], proji[16];
+    uint32_t light_count;
+    float sun_dir[DRK_MAX_LIGHTS][3];
+    float sun_color[DRK_MAX_LIGHTS][3];
+    float sun_cos[DRK_MAX_LIGHTS];
+    float sun_pdf0;            // lights[0]->Pdf value inside the cone
+    float sun_b1[3], sun_b2[3];// SamplingBasis(lights[0]->direction)
+    SkyState sky;
+    const float *env_map;      // RGBA texels for DRK_ENV_MAP
+    uint32_t env_w, env_h;
+    float clip_end;            // Shadow pass far plane
+};
+
 struct RenderParams {
     const drk_octree_node *ob;
     const float *pvol_fog;
     const float *pvol_sdf;
-    const float *env_map;     // RGBA texels for DRK_ENV_MAP
     const float *depth;       // w*h compositing depth or nullptr
+    const KernelConsts *kc;
     uint32_t lvoxc, lvoxc3;
-    float viewi[16], proji[16];
     uint32_t w, h;            // full image size (NDC)
     uint32_t x0, y0, rx, ry;  // tile origin and the rendered extent (even)
     uint32_t tilex;           // output row stride
@@ -104,16 +120,7 @@ struct RenderParams {
     uint32_t phase;
     float hg_g;
     uint32_t env;
-    uint32_t env_w, env_h;
-    uint32_t light_count;
-    float sun_dir[DRK_MAX_LIGHTS][3];
-    float sun_color[DRK_MAX_LIGHTS][3];
-    float sun_cos[DRK_MAX_LIGHTS];
-    float sun_pdf0;            // lights[0]->Pdf value inside the cone
-    float sun_b1[3], sun_b2[3];// SamplingBasis(lights[0]->direction)
-    SkyState sky;
     uint32_t shadow_mode;      // 0: Render, 1: Shadow pass
-    float clip_end;            // Shadow pass far plane
 };
 
 }  // namespace drk

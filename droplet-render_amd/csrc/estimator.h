@@ -57,15 +57,16 @@ struct RayRng {
 // reference's sign flips, ro = viewi.r[3], rd = normalize((posh*proji).xyz0 * viewi).
 // Row-vector convention: v*M = sum_k v[k]*M.r[k].
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void camera_ray(const RenderParams &P, uint32_t px, uint32_t py, F3 &ro, F3 &rd, bool &gm) {
+__device__ __forceinline__ void camera_ray(const KernelConsts &K, uint32_t w, uint32_t h, uint32_t px, uint32_t py,
+                                           F3 &ro, F3 &rd, bool &gm) {
 #pragma clang fp contract(off)
-    float hx = -(2.0f * ((float)px + 0.5f) / (float)P.w - 1.0f);
-    float hy = -(2.0f * ((float)py + 0.5f) / (float)P.h - 1.0f);
-    const float *pi = P.proji;
+    float hx = -(2.0f * ((float)px + 0.5f) / (float)w - 1.0f);
+    float hy = -(2.0f * ((float)py + 0.5f) / (float)h - 1.0f);
+    const float *pi = K.proji;
     float rv[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) rv[j] = hx * pi[0 * 4 + j] + hy * pi[1 * 4 + j] + 0.0f * pi[2 * 4 + j] + 1.0f * pi[3 * 4 + j];
-    const float *vi = P.viewi;
+    const float *vi = K.viewi;
     float d[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) d[j] = rv[0] * vi[0 * 4 + j] + rv[1] * vi[1 * 4 + j] + rv[2] * vi[2 * 4 + j] + 0.0f * vi[3 * 4 + j];
@@ -382,9 +383,18 @@ __device__ __forceinline__ F3 hosek_sky(const SkyState &S, F3 rd) {
     float clph = S.dir[0];
     float rdd = rd.x / rd.y;
     float acr = 1.0f / sqrtf(rdd * rdd + 1.0f);
-    float czp = acr * (rdd * slph + clph);
+    // rd.y == +-0 makes rdd infinite and acr*(...) 0*inf (NaN): take the limit
+    // from the side of the zero's sign, acr*rdd -> sign(rd.x)*sign(rd.y).
+    float czp;
+    if (rd.y == 0.0f) {
+        float sx = rd.x > 0.0f ? 1.0f : (rd.x < 0.0f ? -1.0f : 0.0f);
+        czp = (signbit(rd.y) ? -sx : sx) * slph;
+    } else {
+        czp = acr * (rdd * slph + clph);
+    }
     float cph = sth * slth * czp + cth * clth;
-    float gmma = acosf(cph);
+    // cos(gamma) can exceed 1 by rounding for rd at the sun (acos would be NaN).
+    float gmma = acosf(fmaxf(fminf(cph, 1.0f), -1.0f));
     float gacs = fmaxf(fminf(cph, 1.0f), 0.0f);
     float thcs = fmaxf(rdz, 0.0f);
     float raym = gacs * gacs;
@@ -429,14 +439,14 @@ __device__ __forceinline__ F3 map_env(const float *__restrict__ tex, uint32_t w,
 }
 
 // Environment radiance for an escaping continuation ray.
-__device__ __forceinline__ F3 env_radiance(const RenderParams &P, F3 rd) {
-    if (P.env == DRK_ENV_HOSEK) return hosek_sky(P.sky, rd);
-    if (P.env == DRK_ENV_MAP) return map_env(P.env_map, P.env_w, P.env_h, rd);
+__device__ __forceinline__ F3 env_radiance(uint32_t env, const KernelConsts &K, F3 rd) {
+    if (env == DRK_ENV_HOSEK) return hosek_sky(K.sky, rd);
+    if (env == DRK_ENV_MAP) return map_env(K.env_map, K.env_w, K.env_h, rd);
     return mk3(0.0f, 0.0f, 0.0f);   // NullEnv: pow(0,2.2) = 0
 }
 
 // Sum of SunLight::Evaluate over all lights (src/kernel.cpp:326-327).
-__device__ __forceinline__ F3 lights_eval(const RenderParams &P, F3 rd) {
+__device__ __forceinline__ F3 lights_eval(const KernelConsts &P, F3 rd) {
     F3 s = mk3(0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (uint32_t l = 0; l < DRK_MAX_LIGHTS; ++l) {
@@ -448,13 +458,13 @@ __device__ __forceinline__ F3 lights_eval(const RenderParams &P, F3 rd) {
 }
 
 // SunLight::Pdf of lights[0] (KernelSampler.cpp:326-330).
-__device__ __forceinline__ float sun_pdf(const RenderParams &P, F3 v) {
+__device__ __forceinline__ float sun_pdf(const KernelConsts &P, F3 v) {
     float d = v.x * P.sun_dir[0][0] + v.y * P.sun_dir[0][1] + v.z * P.sun_dir[0][2];
     return d > P.sun_cos[0] ? P.sun_pdf0 : 0.0f;
 }
 
 // SunLight::Sample of lights[0] (KernelSampler.cpp:332-345): cos uniform in [cos angle, 1].
-__device__ __forceinline__ F3 sun_sample(const RenderParams &P, float u1, float u2) {
+__device__ __forceinline__ F3 sun_sample(const KernelConsts &P, float u1, float u2) {
     float ctm = P.sun_cos[0];
     float ct = (1.0f - u1) + u1 * ctm;
     float st = sqrtf(1.0f - ct * ct);

@@ -98,7 +98,7 @@ __global__ void k_hg_sample(float g, const float *iv, const float *u1, const flo
     out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
 }
 
-__global__ void k_sun(RenderParams P, const float *u1, const float *u2, const float *dirs, uint32_t n, float *osd,
+__global__ void k_sun(KernelConsts P, const float *u1, const float *u2, const float *dirs, uint32_t n, float *osd,
                       float *opdf, float *oev) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -167,13 +167,13 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
     count[i] = cnt;
 }
 
-__global__ void k_camera(RenderParams P, const uint32_t *px, const uint32_t *py, uint32_t n, float *ro, float *rd,
-                         uint32_t *gm) {
+__global__ void k_camera(KernelConsts K, uint32_t w, uint32_t h, const uint32_t *px, const uint32_t *py, uint32_t n,
+                         float *ro, float *rd, uint32_t *gm) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     F3 o, d;
     bool g;
-    camera_ray(P, px[i], py[i], o, d, g);
+    camera_ray(K, w, h, px[i], py[i], o, d, g);
     ro[3 * i] = o.x; ro[3 * i + 1] = o.y; ro[3 * i + 2] = o.z;
     rd[3 * i] = d.x; rd[3 * i + 1] = d.y; rd[3 * i + 2] = d.z;
     gm[i] = g ? 1u : 0u;
@@ -294,7 +294,7 @@ int drk_probe_sun(const drk_sun *sun, const float *u1, const float *u2, const fl
     if (!sun || !u1 || !u2 || !dirs || !out_sample_dir || !out_pdf || !out_eval_rgb)
         return set_error(DRK_EINVAL, "drk_probe_sun: NULL argument");
     if (!n) return DRK_OK;
-    RenderParams P;
+    KernelConsts P;
     std::memset(&P, 0, sizeof(P));
     P.light_count = 1;
     for (int c = 0; c < 3; ++c) {
@@ -407,12 +407,10 @@ int drk_probe_camera(const float viewi[16], const float proji[16], uint32_t w, u
     if (!viewi || !proji || !px || !py || !ro || !rd || !gm) return set_error(DRK_EINVAL, "drk_probe_camera: NULL argument");
     if (!w || !h) return set_error(DRK_EINVAL, "drk_probe_camera: empty image");
     if (!n) return DRK_OK;
-    RenderParams P;
-    std::memset(&P, 0, sizeof(P));
-    std::memcpy(P.viewi, viewi, sizeof(P.viewi));
-    std::memcpy(P.proji, proji, sizeof(P.proji));
-    P.w = w;
-    P.h = h;
+    KernelConsts K;
+    std::memset(&K, 0, sizeof(K));
+    std::memcpy(K.viewi, viewi, sizeof(K.viewi));
+    std::memcpy(K.proji, proji, sizeof(K.proji));
     DevBufs B;
     uint32_t *dpx, *dpy, *dgm;
     float *dro, *drd;
@@ -421,7 +419,7 @@ int drk_probe_camera(const float viewi[16], const float proji[16], uint32_t w, u
     PTRY(B.alloc(dro, 3 * (size_t)n), "probe alloc");
     PTRY(B.alloc(drd, 3 * (size_t)n), "probe alloc");
     PTRY(B.alloc(dgm, n), "probe alloc");
-    k_camera<<<blocks(n), kProbeBlock>>>(P, dpx, dpy, n, dro, drd, dgm);
+    k_camera<<<blocks(n), kProbeBlock>>>(K, w, h, dpx, dpy, n, dro, drd, dgm);
     PTRY(hipGetLastError(), "probe launch");
     PTRY(hipMemcpy(ro, dro, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
     PTRY(hipMemcpy(rd, drd, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
 
     uint32_t *stk = stk_all + threadIdx.x;
     const uint32_t stride = kBlock;
+    const KernelConsts &K = *P.kc;
     const float sig_s = P.sigma_s;
     const float sig_e = P.sigma_e;
 
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
 
     auto start_path = [&]() {
         bool gm;
-        camera_ray(P, px, py, ro, rd, gm);
+        camera_ray(K, P.w, P.h, px, py, ro, rd, gm);
         beta = mk3(1, 1, 1);
         pathL = mk3(0, 0, 0);
         pathE = mk3(0, 0, 0);
@@ -143,10 +144,10 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
             // RenderKernel::Shadow (src/kernel.cpp:569-607): a sun-sampled
             // shadow ray from the surface point at the compositing depth.
             kind = RK_SHADOW_PASS;
-            const bool gm1 = gm && pix_depth < P.clip_end;
+            const bool gm1 = gm && pix_depth < K.clip_end;
             const F3 surf = mk3(ro.x + rd.x * pix_depth, ro.y + rd.y * pix_depth, ro.z + rd.z * pix_depth);
             U4 r4 = philox4x32_10(U4{P.sample_base + s_cur, 0u, 0u, kRngShadowPass}, key0, P.key1);
-            rd = sun_sample(P, u01(r4.x), u01(r4.y));
+            rd = sun_sample(K, u01(r4.x), u01(r4.y));
             if (!gm1) { st = ST_END; hit = false; return; }   // untraced lane: unshadowed
             ro = surf;
             start_ray(shadow_ray_id(0));
@@ -299,14 +300,14 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 // SampleVolume at r = scattevs returns the lights' radiance when
                 // the ray escapes (r > 0); Shadow divides by lights[0]'s colour.
                 if (!hit && P.scattevs > 0) {
-                    F3 le = lights_eval(P, rd);
-                    pathL = mk3(le.x / P.sun_color[0][0], le.y / P.sun_color[0][1], le.z / P.sun_color[0][2]);
+                    F3 le = lights_eval(K, rd);
+                    pathL = mk3(le.x / K.sun_color[0][0], le.y / K.sun_color[0][1], le.z / K.sun_color[0][2]);
                 }
                 alpha = 1.0f;
                 path_done = true;
             } else if (kind == RK_SHADOW) {
                 if (!hit) {
-                    F3 le = lights_eval(P, rd);
+                    F3 le = lights_eval(K, rd);
                     pathL.x += shadow_w.x * le.x;
                     pathL.y += shadow_w.y * le.y;
                     pathL.z += shadow_w.z * le.z;
@@ -321,8 +322,8 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 start_ray(cont_ray_id(event));
             } else if (!hit) {
                 if (depth_r > 0) {
-                    F3 lc = lights_eval(P, rd);
-                    F3 le = env_radiance(P, rd);
+                    F3 lc = lights_eval(K, rd);
+                    F3 le = env_radiance(P.env, K, rd);
                     pathL.x += beta.x * lc.x; pathL.y += beta.y * lc.y; pathL.z += beta.z * lc.z;
                     pathE.x += beta.x * le.x; pathE.y += beta.y * le.y; pathE.z += beta.z * le.z;
                 }
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     U4 r4 = philox4x32_10(U4{P.sample_base + s_cur, event, 0u, kRngScatter}, key0, P.key1);
                     float u1 = u01(r4.x), u2 = u01(r4.y), u3 = u01(r4.z), u4 = u01(r4.w);
                     F3 srd, p1, p3;
-                    F3 lrd = sun_sample(P, u3, u4);
+                    F3 lrd = sun_sample(K, u3, u4);
                     if (P.phase == DRK_PHASE_MIE) {
                         srd = mie_sample(mie, rd, u1, u2);
                         p1 = mie_eval_rgb(mie, dot3(srd, rd));
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                         p1 = mk3(h1, h1, h1);
                         p3 = mk3(h3, h3, h3);
                     }
-                    float pd1 = sun_pdf(P, srd), pd2 = sun_pdf(P, lrd);
+                    float pd1 = sun_pdf(K, srd), pd2 = sun_pdf(K, lrd);
                     F3 w1 = mk3(p1.x / (p1.x + pd1), p1.y / (p1.y + pd1), p1.z / (p1.z + pd1));
                     F3 w2 = mk3(p3.x / (p3.x + pd2), p3.y / (p3.y + pd2), p3.z / (p3.z + pd2));
                     F3 x = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                         // The shadow ray's segment carries only its own contribution.
                         wseg = 0.0f;
                         if ((vis >> event) & 1ull) {
-                            F3 le = lights_eval(P, lrd);
+                            F3 le = lights_eval(K, lrd);
                             wseg = gL.x * (shadow_w.x * le.x) + gL.y * (shadow_w.y * le.y) + gL.z * (shadow_w.z * le.z);
                             collected += wseg;
                         }

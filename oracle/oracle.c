@@ -426,9 +426,18 @@ static void sky_eval(const osky *S, const float rd[3], float out[3]) {
     float clth = S->dir[2], slph = S->dir[1], clph = S->dir[0];
     float rdd = rd[0] / rd[1];
     float acr = 1.0f / sqrtf(rdd * rdd + 1.0f);
-    float czp = acr * (rdd * slph + clph);
+    /* Deviation from src/kernel.cpp:336-341, which is NaN here: at rd.y = +-0
+       take the limit acr*rdd -> sign(rd.x)*sign(rd.y), and clamp the acos
+       argument, which rounding can push past 1 for rd at the sun. */
+    float czp;
+    if (rd[1] == 0.0f) {
+        float sx = rd[0] > 0.0f ? 1.0f : (rd[0] < 0.0f ? -1.0f : 0.0f);
+        czp = (signbit(rd[1]) ? -sx : sx) * slph;
+    } else {
+        czp = acr * (rdd * slph + clph);
+    }
     float cph = sth * slth * czp + cth * clth;
-    float gmma = acosf(cph);
+    float gmma = acosf(fmaxf(fminf(cph, 1.0f), -1.0f));
     float gacs = fmaxf(fminf(cph, 1.0f), 0.0f);
     float thcs = fmaxf(rdz, 0.0f);
     float raym = gacs * gacs;

@@ -88,6 +88,11 @@ _SIGS = {
     "drk_kernel_count": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, C.POINTER(C.c_ulonglong)]),
     "drk_kernel_adjoint": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _F, _F, _F]),
     "drk_kernel_adjoint_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P]),
+    "drk_kernel_record_bytes": (C.c_int, [_P, u32, u32, u32, u32, u32, C.POINTER(C.c_size_t)]),
+    "drk_kernel_render_record_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, C.c_size_t,
+                                                  _P]),
+    "drk_kernel_adjoint_record_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P,
+                                                   C.c_size_t, _P]),
     "drk_synth_cloud": (C.c_int, [u32, u32, u64, _F, u32, _F]),
     "drk_probe_philox": (C.c_int, [_U, _U, u32, _U]),
     "drk_probe_ray_stream": (C.c_int, [_U, _U, _U, u32, u32, u32, _F]),

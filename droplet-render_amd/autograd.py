@@ -2,10 +2,14 @@
 
 `render(kernel, density, ...)` returns the two phb sum images for FOG brick
 values `density` (a CUDA tensor in the layout of Scene.pvol[FOG]); its
-backward runs the path-replay adjoint kernel (drk_kernel_adjoint_device) with
-the same RNG keys, so d(image)/d(density) is differentiated for the very
-samples the forward drew. The octree's majorants stay fixed, as the adjoint
-assumes: keep the densities at or below them (`Scene.raise_majorants`).
+backward runs the path-replay adjoint kernel with the same RNG keys, so
+d(image)/d(density) is differentiated for the very samples the forward drew.
+With record=True (the default) the forward keeps each path's radiance and
+shadow-escape bits (40 bytes per pixel sample, drk_kernel_render_record_device)
+until backward, which then traces every path once instead of twice
+(drk_kernel_adjoint_record_device); record=False keeps no per-sample state.
+The octree's majorants stay fixed, as the adjoint assumes: keep the densities
+at or below them (`Scene.raise_majorants`).
 """
 import torch
 
@@ -14,7 +18,7 @@ from . import _lib as L
 
 class _Render(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, density, kernel, x0, y0, tilex, tiley, samples, seed, sample_base):
+    def forward(ctx, density, kernel, x0, y0, tilex, tiley, samples, seed, sample_base, record):
         scene = kernel.scene
         assert density.is_cuda and density.dtype == torch.float32 and density.is_contiguous()
         assert density.numel() == scene.leafx[1] * scene.lvoxc3
@@ -24,7 +28,14 @@ class _Render(torch.autograd.Function):
         out_l = torch.empty(n, dtype=torch.float32, device=density.device)
         out_e = torch.empty(n, dtype=torch.float32, device=density.device)
         stream = torch.cuda.current_stream(density.device)
-        kernel.render_device(x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, stream=stream)
+        ctx.rec = None
+        if record:
+            nb = kernel.record_bytes(x0, y0, tilex, tiley, samples)
+            ctx.rec = torch.empty((nb + 15) // 16 * 4, dtype=torch.float32, device=density.device)
+            kernel.render_record_device(x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, ctx.rec,
+                                        stream=stream)
+        else:
+            kernel.render_device(x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, stream=stream)
         ctx.kernel = kernel
         ctx.args = (x0, y0, tilex, tiley, samples, seed, sample_base)
         ctx.save_for_backward(density)
@@ -41,17 +52,22 @@ class _Render(torch.autograd.Function):
         g_e = torch.zeros(tiley, tilex, 4, device=density.device) if g_e is None else g_e.contiguous().float()
         grad = torch.zeros_like(density)
         stream = torch.cuda.current_stream(density.device)
-        kernel.adjoint_device(x0, y0, tilex, tiley, samples, seed, sample_base, g_l, g_e, grad, stream=stream)
-        return grad, None, None, None, None, None, None, None, None
+        if ctx.rec is not None:
+            kernel.adjoint_record_device(x0, y0, tilex, tiley, samples, seed, sample_base, g_l, g_e, grad, ctx.rec,
+                                         stream=stream)
+            ctx.rec = None
+        else:
+            kernel.adjoint_device(x0, y0, tilex, tiley, samples, seed, sample_base, g_l, g_e, grad, stream=stream)
+        return grad, None, None, None, None, None, None, None, None, None
 
 
-def render(kernel, density, samples=None, seed=0, sample_base=0, x0=0, y0=0, tilex=None, tiley=None):
+def render(kernel, density, samples=None, seed=0, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, record=True):
     """Differentiable render: (lights sums, environment sums), each (tiley, tilex, 4)."""
     tilex = tilex or kernel.tilex
     tiley = tiley or kernel.tiley
     samples = samples or 1
     return _Render.apply(density, kernel, int(x0), int(y0), int(tilex), int(tiley), int(samples), int(seed),
-                         int(sample_base))
+                         int(sample_base), bool(record))
 
 
 __all__ = ["render", "L"]

@@ -46,6 +46,7 @@ using namespace drk;
 struct drk_scene {
     std::vector<drk_octree_node> h_ob;
     drk_octree_node *d_ob = nullptr;
+    uint32_t *d_chx = nullptr;      // per node 8 child links: index | kChildLeaf if the child is a leaf
     float *d_fog = nullptr;
     float *d_sdf = nullptr;
     uint32_t node_count = 0;
@@ -70,6 +71,11 @@ struct drk_kernel {
     float *d_env = nullptr;
     KernelConsts C;                 // host copy of the per-kernel constants
     KernelConsts *d_kc = nullptr;   // device copy read by the kernels
+    uint32_t bands = 1;             // work bands (one per XCD)
+    uint32_t xcc = 1;               // XCDs of the device
+    uint32_t grad_copies = 1;       // gradient copies of the adjoint (one per XCD by default)
+    float *d_gradx = nullptr;       // grad_copies * (FOG gradient size) floats
+    size_t gradx_bytes = 0;
     float *d_depth = nullptr;
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
@@ -150,6 +156,21 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipMalloc(&s->d_ob, sizeof(drk_octree_node) * node_count);
     if (e == hipSuccess) e = hipMemcpy(s->d_ob, ob, sizeof(drk_octree_node) * node_count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        // Child links with the child's leaf flag folded in, so a traversal
+        // step reads one word instead of the child index and then the child.
+        std::vector<uint32_t> chx((size_t)node_count * 8, 0u);
+        for (uint32_t i = 0; i < node_count; ++i)
+            for (int c = 0; c < 8; ++c) {
+                uint32_t ch = ob[i].chn[c];
+                if (!ch) continue;
+                const drk_octree_node &m = ob[ch];
+                bool leaf = m.volx[0] != DRK_NO_BRICK || m.volx[1] != DRK_NO_BRICK;
+                chx[(size_t)i * 8 + c] = ch | (leaf ? kChildLeaf : 0u);
+            }
+        e = hipMalloc(&s->d_chx, chx.size() * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemcpy(s->d_chx, chx.data(), chx.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
     size_t nb = (size_t)leafx_fog * lvoxc * lvoxc * lvoxc;
     if (e == hipSuccess && nb) e = hipMalloc(&s->d_fog, nb * sizeof(float));
     if (e == hipSuccess && nb) e = hipMemcpy(s->d_fog, pvol_fog, nb * sizeof(float), hipMemcpyHostToDevice);
@@ -175,10 +196,12 @@ int drk_scene_update_fog(drk_scene *s, const float *pvol_fog, const float *qval_
 }
 
 const drk_octree_node *drk_internal_scene_nodes(const drk_scene *s) { return s ? s->d_ob : nullptr; }
+const uint32_t *drk_internal_scene_links(const drk_scene *s) { return s ? s->d_chx : nullptr; }
 
 void drk_scene_destroy(drk_scene *s) {
     if (!s) return;
     if (s->d_ob) (void)hipFree(s->d_ob);
+    if (s->d_chx) (void)hipFree(s->d_chx);
     if (s->d_fog) (void)hipFree(s->d_fog);
     if (s->d_sdf) (void)hipFree(s->d_sdf);
     delete s;
@@ -230,6 +253,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     k->tilex = tilex;
     k->tiley = tiley;
     P.ob = scene->d_ob;
+    P.chx = scene->d_chx;
     P.pvol_fog = scene->d_fog;
     P.pvol_sdf = scene->d_sdf;
     P.lvoxc = scene->lvoxc;
@@ -283,7 +307,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
     size_t tile_bytes = (size_t)tilex * tiley * 16;
     if (e == hipSuccess) e = hipMalloc(&k->d_out, 4 * tile_bytes);
-    if (e == hipSuccess) e = hipMalloc(&k->d_counter, 256);
+    if (e == hipSuccess) e = hipMalloc(&k->d_counter, kMaxBands * kCounterStride * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&k->d_counts, 64);
     if (e == hipSuccess && smp->env == DRK_ENV_MAP) {
         size_t eb = (size_t)smp->env_w * smp->env_h * 16;
@@ -316,6 +340,18 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     if (drk_device_info(nullptr, 0, &cus) != DRK_OK || cus == 0) cus = 1;
     k->cu_count = cus;
     k->blocks_per_cu = (uint32_t)render_blocks_per_cu(kModeForwardSq);
+    {
+        // One adjoint gradient copy per XCD (each XCD has a private L2).
+        int dev = 0, xcc = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || xcc < 1)
+            xcc = 1;
+        k->xcc = (uint32_t)xcc;
+        k->grad_copies = (uint32_t)xcc;
+        if (const char *g = std::getenv("DRK_GRAD_COPIES")) k->grad_copies = std::max(1, std::atoi(g));
+        k->bands = (uint32_t)std::min<int>(xcc, (int)kMaxBands);
+        if (const char *b = std::getenv("DRK_BANDS")) k->bands = std::max(1u, std::min<uint32_t>(std::atoi(b), kMaxBands));
+    }
     if (const char *c = std::getenv("DRK_CHUNK")) k->chunk_override = (uint32_t)std::atoi(c);
     if (const char *g = std::getenv("DRK_GRID")) k->grid_override = (uint32_t)std::atoi(g);
     *out = k;
@@ -342,9 +378,18 @@ struct AdjointArgs {
     float *grad = nullptr;
 };
 
+// Rendered extent of a call: the tile rounded down to even sizes (the
+// reference renders 2x2 packets, src/kernel.cpp:453) and clipped to the image.
+static void rendered_extent(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                            uint32_t &rx, uint32_t &ry) {
+    rx = x0 < k->P.w ? std::min(tilex & ~1u, k->P.w - x0) : 0u;
+    ry = y0 < k->P.h ? std::min(tiley & ~1u, k->P.h - y0) : 0u;
+}
+
 static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
-                       hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr) {
+                       hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr,
+                       float *rec = nullptr) {
     if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
         return set_error(DRK_EINVAL, "render: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley, k->tilex, k->tiley);
     if (samples == 0) return set_error(DRK_EINVAL, "render: samples must be > 0");
@@ -363,10 +408,10 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     P.shadow_mode = shadow ? 1u : 0u;
     P.depth = (shadow || k->depthcomp) ? k->d_depth : nullptr;
     const bool sumsq = d_sql && d_sqe;
-    const int mode = adj ? kModeAdjoint : count ? kModeCount : sumsq ? kModeForwardSq : kModeForward;
+    const int mode = adj ? kModeAdjoint : count ? kModeCount : rec ? kModeRecord : sumsq ? kModeForwardSq : kModeForward;
     // Clip the rendered extent to the image (gm mask, src/kernel.cpp:472-474).
-    uint32_t rx = P.x0 < P.w ? std::min(P.rx, P.w - P.x0) : 0u;
-    uint32_t ry = P.y0 < P.h ? std::min(P.ry, P.h - P.y0) : 0u;
+    uint32_t rx, ry;
+    rendered_extent(k, x0, y0, tilex, tiley, rx, ry);
     P.rx = rx;
     P.ry = ry;
     // Pixels outside the rendered extent (odd tiles, or outside the image)
@@ -400,6 +445,9 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     WorkParams W;
     std::memset(&W, 0, sizeof(W));
     W.counter = k->d_counter;
+    W.grad_copies = 1;
+    W.grad_nxcc = k->xcc;
+    W.nbands = k->bands;
     W.npix = npix;
     W.chunk = chunk;
     W.nchunks = nchunks;
@@ -409,10 +457,30 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     W.sq_l = sumsq ? d_sql : nullptr;
     W.sq_e = sumsq ? d_sqe : nullptr;
     W.counts = k->d_counts;
+    W.rec = rec;
+    uint32_t copies = 1;
+    const size_t gn = (size_t)k->scene->leafx[1] * k->P.lvoxc3;
     if (adj) {
         W.adj_l = adj->adj_l;
         W.adj_e = adj->adj_e;
         W.grad = adj->grad;
+        // Per-XCD copies while they stay small next to HBM (<= 4 GiB).
+        copies = k->grad_copies;
+        while (copies > 1 && (double)copies * gn * sizeof(float) > 4.0 * (1ull << 30)) copies >>= 1;
+        if (copies > 1) {
+            size_t need = (size_t)copies * gn * sizeof(float);
+            if (need > k->gradx_bytes) {
+                if (k->d_gradx) (void)hipFree(k->d_gradx);
+                k->d_gradx = nullptr;
+                k->gradx_bytes = 0;
+                HIP_TRY(hipMalloc(&k->d_gradx, need), "adjoint: gradient copies");
+                k->gradx_bytes = need;
+            }
+            HIP_TRY(launch_fill(k->d_gradx, (size_t)copies * gn, 0.0f, s), "adjoint: clear");
+            W.grad = k->d_gradx;
+            W.grad_copies = copies;
+            W.grad_stride = gn;
+        }
     }
     if (nchunks > 1 && !adj) {
         size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
@@ -439,6 +507,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         k->ev_used.emplace_back(e0, e1);
     }
     HIP_TRY(launch_render(P, W, grid, mode, s, e0, e1), "render: launch");
+    if (adj && copies > 1) HIP_TRY(launch_grad_sum(k->d_gradx, copies, gn, adj->grad, s), "adjoint: gradient sum");
     return DRK_OK;
 }
 
@@ -451,6 +520,38 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t t
     hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
     return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, d_sumsq_lights,
                        d_sumsq_env, s, false);
+}
+
+int drk_kernel_record_bytes(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                            uint32_t samples, size_t *bytes) {
+    if (!k || !bytes) return set_error(DRK_EINVAL, "drk_kernel_record_bytes: NULL argument");
+    uint32_t rx, ry;
+    rendered_extent(k, x0, y0, tilex, tiley, rx, ry);
+    *bytes = (size_t)rx * ry * samples * kRecordBytes;
+    return DRK_OK;
+}
+
+static int check_record(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                        uint32_t samples, const void *rec, size_t rec_bytes, const char *fn) {
+    if (!rec) return set_error(DRK_EINVAL, "%s: NULL record", fn);
+    if ((uintptr_t)rec % 16) return set_error(DRK_EINVAL, "%s: record must be 16-byte aligned", fn);
+    size_t need = 0;
+    drk_kernel_record_bytes(k, x0, y0, tilex, tiley, samples, &need);
+    if (rec_bytes < need) return set_error(DRK_EINVAL, "%s: record holds %zu bytes, needs %zu", fn, rec_bytes, need);
+    return DRK_OK;
+}
+
+int drk_kernel_render_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                    uint32_t samples, uint64_t seed, uint32_t sample_base, float *d_lights,
+                                    float *d_env, void *d_record, size_t record_bytes, void *stream) {
+    if (!k || !d_lights || !d_env) return set_error(DRK_EINVAL, "drk_kernel_render_record_device: NULL argument");
+    if (k->P.scattevs > 64)
+        return set_error(DRK_EINVAL, "drk_kernel_render_record_device: scattevs > 64 is not supported");
+    int rc = check_record(k, x0, y0, tilex, tiley, samples, d_record, record_bytes, "drk_kernel_render_record_device");
+    if (rc != DRK_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
+    return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, nullptr, nullptr, s,
+                       false, false, nullptr, static_cast<float *>(d_record));
 }
 
 int drk_kernel_render(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples) {
@@ -483,21 +584,38 @@ int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, ui
     return DRK_OK;
 }
 
-int drk_kernel_adjoint_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
-                              uint32_t samples, uint64_t seed, uint32_t sample_base, const float *d_adj_lights,
-                              const float *d_adj_env, float *d_grad_fog, void *stream) {
-    if (!k || !d_adj_lights || !d_adj_env || !d_grad_fog)
-        return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: NULL argument");
-    if (k->P.scattevs > 64) return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: scattevs > 64 is not supported");
-    if (k->depthcomp) return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: depth compositing is not differentiated");
-    if (k->scene->leafx[1] == 0) return set_error(DRK_EINVAL, "drk_kernel_adjoint_device: scene has no FOG bricks");
+static int adjoint_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+                        uint64_t seed, uint32_t sample_base, const float *d_adj_lights, const float *d_adj_env,
+                        float *d_grad_fog, const void *rec, void *stream, const char *fn) {
+    if (!k || !d_adj_lights || !d_adj_env || !d_grad_fog) return set_error(DRK_EINVAL, "%s: NULL argument", fn);
+    if (k->P.scattevs > 64) return set_error(DRK_EINVAL, "%s: scattevs > 64 is not supported", fn);
+    if (k->depthcomp) return set_error(DRK_EINVAL, "%s: depth compositing is not differentiated", fn);
+    if (k->scene->leafx[1] == 0) return set_error(DRK_EINVAL, "%s: scene has no FOG bricks", fn);
     AdjointArgs A;
     A.adj_l = d_adj_lights;
     A.adj_e = d_adj_env;
     A.grad = d_grad_fog;
     hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
     return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, nullptr, nullptr, nullptr, nullptr, s,
-                       false, false, &A);
+                       false, false, &A, const_cast<float *>(static_cast<const float *>(rec)));
+}
+
+int drk_kernel_adjoint_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                              uint32_t samples, uint64_t seed, uint32_t sample_base, const float *d_adj_lights,
+                              const float *d_adj_env, float *d_grad_fog, void *stream) {
+    return adjoint_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_adj_lights, d_adj_env, d_grad_fog,
+                        nullptr, stream, "drk_kernel_adjoint_device");
+}
+
+int drk_kernel_adjoint_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                     uint32_t samples, uint64_t seed, uint32_t sample_base,
+                                     const float *d_adj_lights, const float *d_adj_env, float *d_grad_fog,
+                                     const void *d_record, size_t record_bytes, void *stream) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_adjoint_record_device: NULL kernel");
+    int rc = check_record(k, x0, y0, tilex, tiley, samples, d_record, record_bytes, "drk_kernel_adjoint_record_device");
+    if (rc != DRK_OK) return rc;
+    return adjoint_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_adj_lights, d_adj_env, d_grad_fog,
+                        d_record, stream, "drk_kernel_adjoint_record_device");
 }
 
 int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
@@ -582,6 +700,7 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_counts) (void)hipFree(k->d_counts);
     if (k->d_env) (void)hipFree(k->d_env);
     if (k->d_kc) (void)hipFree(k->d_kc);
+    if (k->d_gradx) (void)hipFree(k->d_gradx);
     if (k->d_depth) (void)hipFree(k->d_depth);
     for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
     for (auto &pr : k->ev_free) {

@@ -102,8 +102,12 @@ struct KernelConsts {
     float clip_end;            // Shadow pass far plane
 };
 
+// Child link word: node index (bits 23:0) and the child's leaf flag.
+constexpr uint32_t kChildLeaf = 0x80000000u;
+
 struct RenderParams {
     const drk_octree_node *ob;
+    const uint32_t *chx;      // node_count x 8 child links (index | kChildLeaf)
     const float *pvol_fog;
     const float *pvol_sdf;
     const float *depth;       // w*h compositing depth or nullptr

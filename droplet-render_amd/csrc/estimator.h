@@ -164,7 +164,7 @@ constexpr uint32_t kStkNodeMask = 0x00ffffffu;
 
 // Advances to the next leaf in ray order. Returns false when exhausted.
 // `touches` counts node-record reads (for the algorithmic-bytes model).
-__device__ __forceinline__ bool trav_next(const drk_octree_node *ob, Traversal &T, uint32_t *stk, uint32_t stride,
+__device__ __forceinline__ bool trav_next(const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk, uint32_t stride,
                                           uint32_t &leaf, float &tra, float &trb, uint32_t &touches) {
     for (;;) {
         if (T.lvl < 0) return false;
@@ -188,11 +188,12 @@ __device__ __forceinline__ bool trav_next(const drk_octree_node *ob, Traversal &
         child_interval(T.t0, T.t1, tn, ct0, ct1);
         uint32_t nxt = (tn == 7) ? 8u : next_node(ct1, tn);
         stk[T.lvl * stride] = node | (nxt << 24) | (tn << 28);
-        uint32_t ch = ob[node].chn[build_slot(tn ^ T.a)];
+        const uint32_t link = chx[node * 8u + build_slot(tn ^ T.a)];
+        const uint32_t ch = link & kStkNodeMask;
         touches += 1;
         if (ct1.x < 0.0f || ct1.y < 0.0f || ct1.z < 0.0f || ch == 0) continue;
         touches += 1;
-        if (is_leaf(ob[ch])) {
+        if (link & kChildLeaf) {
             leaf = ch;
             tra = fmaxf(fmaxf(ct0.x, ct0.y), ct0.z);
             trb = fminf(fminf(ct1.x, ct1.y), ct1.z);

@@ -122,7 +122,8 @@ __global__ void k_trilinear(const float *brick, uint32_t lvoxc, float4 ce, const
     if (i < n) out[i] = brick_lookup(brick, mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), ce, lvoxc);
 }
 
-__global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node *ob, const float *ro, const float *rd,
+__global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node *ob, const uint32_t *chx,
+                                                          const float *ro, const float *rd,
                                                           uint32_t n, uint32_t max_leaves, uint32_t *nodes,
                                                           float *t0o, float *t1o, uint32_t *count) {
     __shared__ uint32_t stk_all[kMaxDepth * kProbeBlock];
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
             T.lvl = 0;
             uint32_t leaf;
             float a0, a1;
-            while (trav_next(ob, T, stk, kProbeBlock, leaf, a0, a1, touches)) {
+            while (trav_next(chx, T, stk, kProbeBlock, leaf, a0, a1, touches)) {
                 if (cnt < max_leaves) {
                     nodes[(size_t)i * max_leaves + cnt] = leaf;
                     t0o[(size_t)i * max_leaves + cnt] = a0;
@@ -198,6 +199,7 @@ using namespace drk;
 
 // The scene's device octree lives in the opaque handle; api.cpp exposes it here.
 extern "C" const drk_octree_node *drk_internal_scene_nodes(const drk_scene *s);
+extern "C" const uint32_t *drk_internal_scene_links(const drk_scene *s);
 
 extern "C" {
 
@@ -391,7 +393,7 @@ int drk_probe_traverse(const drk_scene *scene, const float *ro, const float *rd,
     PTRY(B.alloc(dt0, (size_t)n * max_leaves), "probe alloc");
     PTRY(B.alloc(dt1, (size_t)n * max_leaves), "probe alloc");
     PTRY(B.alloc(dc, n), "probe alloc");
-    k_traverse<<<blocks(n), kProbeBlock>>>(drk_internal_scene_nodes(scene), dro, drd, n, max_leaves, dn, dt0, dt1, dc);
+    k_traverse<<<blocks(n), kProbeBlock>>>(drk_internal_scene_nodes(scene), drk_internal_scene_links(scene), dro, drd, n, max_leaves, dn, dt0, dt1, dc);
     PTRY(hipGetLastError(), "probe launch");
     if (max_leaves) {
         PTRY(hipMemcpy(nodes, dn, 4 * (size_t)n * max_leaves, hipMemcpyDeviceToHost), "probe download");

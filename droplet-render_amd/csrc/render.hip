@@ -4,9 +4,13 @@
 // and runs a flat state machine (find next leaf -> one Woodcock step -> ray
 // end / scattering event), so lanes never wait for each other at leaf, ray
 // or sample boundaries. A lane that finishes its work unit (one pixel x a
-// chunk of samples) refills from a global unit counter with one
-// wave-aggregated atomic (ballot + mbcnt prefix), which keeps the wave's
-// lanes busy until the whole tile is drained.
+// chunk of samples) refills from a unit counter with one wave-aggregated
+// atomic (ballot + mbcnt prefix), which keeps the wave's lanes busy until the
+// whole tile is drained. The tile's pixels are split into one row band per
+// XCD, each with its own counter: the waves of an XCD work on their own band
+// while it lasts (its rays see a smaller part of the volume, so the density
+// and gradient bricks they touch stay in that XCD's L2), then help drain the
+// others.
 //
 // The estimator is the reference's SampleVolume recursion
 // (src/kernel.cpp:128-445) unrolled into a loop: each scattering event
@@ -20,7 +24,10 @@
 // tracked segment with downstream adjoint radiance W, adds W * w_v(x)/rho(x)
 // at the real collision and -W * sigma_e * integral of w_v over the leaf
 // intervals the tracker covered (one uniform point per leaf, from a separate
-// replayable stream). The majorants are held fixed. No path is stored.
+// replayable stream). The majorants are held fixed. No path vertex is
+// stored; a forward launch in record mode may keep each path's radiance and
+// shadow-escape bits (40 B per sample), and an adjoint launch given that
+// record starts at pass 1.
 #include <hip/hip_runtime.h>
 
 #include "drk_common.h"
@@ -35,6 +42,22 @@ enum : uint32_t { ST_WORK = 0, ST_TRAV = 1, ST_LEAF = 2, ST_END = 3, ST_DONE = 4
 enum : uint32_t { RK_PRIMARY = 0, RK_CONT = 1, RK_SHADOW = 2, RK_SHADOW_PASS = 3 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// The XCD (accelerator complex die) the wave runs on: hardware register
+// XCC_ID (id 20), bits 3:0, on gfx942/gfx950. Each XCD has its own L2, so
+// float atomics into a per-XCD copy of the gradient never contend for a
+// cache line across dies; a wave stays on its XCD for its lifetime.
+__device__ __forceinline__ uint32_t xcc_id() {
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+}
+
+// The wave's compute unit within its XCD: hardware register HW_ID (id 4),
+// CU_ID bits 11:8 and SE_ID bits 14:13 on gfx942/gfx950.
+__device__ __forceinline__ uint32_t cu_slot() {
+    uint32_t cu = __builtin_amdgcn_s_getreg((3 << 11) | (8 << 6) | 4);
+    uint32_t se = __builtin_amdgcn_s_getreg((1 << 11) | (13 << 6) | 4);
+    return cu | (se << 4);
+}
 
 // Wave-aggregated fetch of work units for the lanes that need one.
 __device__ __forceinline__ uint32_t fetch_units(uint32_t *counter, bool need) {
@@ -53,6 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
     constexpr bool kSumSq = (kMode == kModeForwardSq || kMode == kModeCount);
     constexpr bool kCount = (kMode == kModeCount);
     constexpr bool kAdj = (kMode == kModeAdjoint);
+    constexpr bool kRec = (kMode == kModeRecord);
     __shared__ MieLds mie;
     __shared__ uint32_t stk_all[kMaxDepth * kBlock];
     load_mie(mie);
@@ -66,7 +90,9 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
 
     // Work unit.
     uint32_t st = ST_WORK;
-    uint32_t unit = 0, pix_local = 0, px = 0, py = 0, key0 = 0;
+    uint32_t chunk_cur = 0, pix_local = 0, px = 0, py = 0, key0 = 0;
+    const uint32_t band0 = xcc_id();   // first band: this wave's XCD
+    uint32_t band_step = 0;            // bands this wave has found drained (wave-uniform)
     uint32_t s_cur = 0, s_end = 0;
     float pix_depth = 3.402823466e38f;
     float accL[3] = {0, 0, 0}, accE[3] = {0, 0, 0}, accA = 0.0f;
@@ -97,6 +123,9 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
     uint64_t vis = 0;
     RayRng arng;
     arng.s0 = arng.s1 = arng.s2 = arng.s3 = 1u;
+    // Gradient copy of this wave: spread over XCDs first, then over CUs.
+    float *const grad_copy =
+        kAdj ? W.grad + (size_t)((xcc_id() + W.grad_nxcc * cu_slot()) % W.grad_copies) * W.grad_stride : nullptr;
 
     auto start_ray = [&](uint32_t ray_id) {
         td = 0.0f;
@@ -157,6 +186,25 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
         start_ray(0);
     };
 
+    // Adjoint: set up the next sample of the current pixel. With a path
+    // record, its total and shadow-escape bits come from the forward launch
+    // and the replay starts directly at pass 1.
+    auto adj_begin = [&]() {
+        if (W.rec) {
+            const size_t n = (size_t)W.npix * P.samples;
+            const size_t ri = (size_t)s_cur * W.npix + pix_local;
+            const float4 *R = reinterpret_cast<const float4 *>(W.rec);
+            float4 a = R[ri], b = R[n + ri];
+            uint2 v = reinterpret_cast<const uint2 *>(R + 2 * n)[ri];
+            ltot = gL.x * a.x + gL.y * a.y + gL.z * a.z + gE.x * b.x + gE.y * b.y + gE.z * b.z + gA * a.w;
+            vis = ((uint64_t)v.y << 32) | (uint64_t)v.x;
+            pass = 1;
+        } else {
+            pass = 0;
+            vis = 0;
+        }
+    };
+
     // Adds weight * (trilinear weight of vertex v at x) to every corner v of
     // the current leaf's gradient brick (adjoint pass 1 only).
     auto grad_at = [&](F3 x, float weight) {
@@ -164,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
         uint32_t idx[8];
         float w8[8];
         brick_weights(x, ce, P.lvoxc, idx, w8);
-        float *g = W.grad + brick_off;
+        float *g = grad_copy + brick_off;
 #pragma unroll
         for (int v = 0; v < 8; ++v) unsafeAtomicAdd(g + idx[v], weight * w8[v]);
     };
@@ -181,45 +229,53 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
 
     for (;;) {
         // ---- refill lanes that finished their unit ------------------------
+        // The pixels are split into W.nbands contiguous row bands; a wave
+        // drains the band of its own XCD first, then the others in turn.
         bool need = (st == ST_WORK);
-        if (__ballot(need)) {
-            uint32_t u = fetch_units(W.counter, need);
-            if (need) {
-                if (u >= W.total_units) {
-                    st = ST_DONE;
-                } else {
-                    unit = u;
-                    uint32_t chunk = u / W.npix;
-                    pix_local = u - chunk * W.npix;
-                    uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
-                    px = P.x0 + lx;
-                    py = P.y0 + ly;
-                    key0 = py * P.w + px;
-                    s_cur = chunk * W.chunk;
-                    s_end = min(s_cur + W.chunk, P.samples);
-                    pix_depth = P.depth ? P.depth[key0] : 3.402823466e38f;
-                    accL[0] = accL[1] = accL[2] = 0.0f;
-                    accE[0] = accE[1] = accE[2] = 0.0f;
-                    accA = 0.0f;
-                    sqL[0] = sqL[1] = sqL[2] = 0.0f;
-                    sqE[0] = sqE[1] = sqE[2] = 0.0f;
-                    if (kAdj) {
-                        size_t d = (size_t)ly * P.tilex + lx;
-                        float4 a = reinterpret_cast<const float4 *>(W.adj_l)[d];
-                        float4 b = reinterpret_cast<const float4 *>(W.adj_e)[d];
-                        gL = mk3(a.x, a.y, a.z);
-                        gE = mk3(b.x, b.y, b.z);
-                        gA = a.w + b.w;   // alpha appears in both buffers
-                        pass = 0;
-                        vis = 0;
-                        if (gL.x == 0.0f && gL.y == 0.0f && gL.z == 0.0f && gE.x == 0.0f && gE.y == 0.0f &&
-                            gE.z == 0.0f && gA == 0.0f)
-                            s_cur = s_end;   // no adjoint on this pixel: nothing to differentiate
-                    }
-                    if (s_cur < s_end) start_path();
-                    else st = ST_WORK;
+        while (__ballot(need)) {
+            if (band_step >= W.nbands) {
+                if (need) st = ST_DONE;   // every band drained
+                break;
+            }
+            const uint32_t b = (band0 + band_step) % W.nbands;
+            const uint32_t p0 = (uint32_t)((uint64_t)b * W.npix / W.nbands);
+            const uint32_t bp = (uint32_t)((uint64_t)(b + 1) * W.npix / W.nbands) - p0;
+            const uint32_t u = fetch_units(W.counter + kCounterStride * b, need);
+            if (need && u < bp * W.nchunks) {
+                chunk_cur = u / bp;
+                pix_local = p0 + (u - chunk_cur * bp);
+                uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
+                px = P.x0 + lx;
+                py = P.y0 + ly;
+                key0 = py * P.w + px;
+                s_cur = chunk_cur * W.chunk;
+                s_end = min(s_cur + W.chunk, P.samples);
+                pix_depth = P.depth ? P.depth[key0] : 3.402823466e38f;
+                accL[0] = accL[1] = accL[2] = 0.0f;
+                accE[0] = accE[1] = accE[2] = 0.0f;
+                accA = 0.0f;
+                sqL[0] = sqL[1] = sqL[2] = 0.0f;
+                sqE[0] = sqE[1] = sqE[2] = 0.0f;
+                if (kAdj) {
+                    size_t d = (size_t)ly * P.tilex + lx;
+                    float4 a = reinterpret_cast<const float4 *>(W.adj_l)[d];
+                    float4 c = reinterpret_cast<const float4 *>(W.adj_e)[d];
+                    gL = mk3(a.x, a.y, a.z);
+                    gE = mk3(c.x, c.y, c.z);
+                    gA = a.w + c.w;   // alpha appears in both buffers
+                    if (gL.x == 0.0f && gL.y == 0.0f && gL.z == 0.0f && gE.x == 0.0f && gE.y == 0.0f &&
+                        gE.z == 0.0f && gA == 0.0f)
+                        s_cur = s_end;   // no adjoint on this pixel: nothing to differentiate
+                    else
+                        adj_begin();
+                }
+                if (s_cur < s_end) {
+                    start_path();
+                    need = false;
                 }
             }
+            // Lanes that drew past the band's end leave it for the next one.
+            if (__ballot(need && !(u < bp * W.nchunks))) ++band_step;
         }
         if (!__ballot(st != ST_DONE)) break;
 
@@ -235,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 T.lvl = -1;
                 found = true;
             } else {
-                found = trav_next(P.ob, T, stk, stride, leaf, tra, trbn, cnt_nodes);
+                found = trav_next(P.chx, T, stk, stride, leaf, tra, trbn, cnt_nodes);
             }
             // Compositing depth stops primary rays (src/kernel.cpp:217-222).
             if (found && kind == RK_PRIMARY && !(td < pix_depth)) found = false;
@@ -311,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     pathL.x += shadow_w.x * le.x;
                     pathL.y += shadow_w.y * le.y;
                     pathL.z += shadow_w.z * le.z;
-                    if (kAdj && pass == 0) vis |= 1ull << event;
+                    if ((kAdj && pass == 0) || kRec) vis |= 1ull << event;
                 }
                 // Continue the path along the phase-sampled direction.
                 rd = cont_dir;
@@ -390,13 +446,24 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     path_done = false;
                 }
                 if (path_done) {
-                    pass = 0;
-                    vis = 0;
                     ++s_cur;
-                    if (s_cur < s_end) start_path();
-                    else st = ST_WORK;
+                    if (s_cur < s_end) {
+                        adj_begin();
+                        start_path();
+                    } else {
+                        st = ST_WORK;
+                    }
                 }
             } else if (path_done) {
+                if (kRec) {
+                    const size_t n = (size_t)W.npix * P.samples;
+                    const size_t ri = (size_t)s_cur * W.npix + pix_local;
+                    float4 *R = reinterpret_cast<float4 *>(W.rec);
+                    R[ri] = make_float4(pathL.x, pathL.y, pathL.z, alpha);
+                    R[n + ri] = make_float4(pathE.x, pathE.y, pathE.z, 0.0f);
+                    reinterpret_cast<uint2 *>(R + 2 * n)[ri] = make_uint2((uint32_t)vis, (uint32_t)(vis >> 32));
+                    vis = 0;
+                }
                 accL[0] += pathL.x; accL[1] += pathL.y; accL[2] += pathL.z;
                 accE[0] += pathE.x; accE[1] += pathE.y; accE[2] += pathE.z;
                 accA += alpha;
@@ -409,8 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                     start_path();
                 } else {
                     // Unit complete: write its sums.
-                    uint32_t chunk = unit / W.npix;
-                    size_t o = (size_t)chunk * W.npix + pix_local;
+                    size_t o = (size_t)chunk_cur * W.npix + pix_local;
                     float4 *oL = W.partial ? reinterpret_cast<float4 *>(W.partial) : nullptr;
                     if (W.partial) {
                         size_t plane = (size_t)W.nchunks * W.npix;
@@ -482,6 +548,7 @@ int render_blocks_per_cu(int mode) {
     case kModeForward: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeForward>, kBlock, 0); break;
     case kModeForwardSq: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeForwardSq>, kBlock, 0); break;
     case kModeCount: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeCount>, kBlock, 0); break;
+    case kModeRecord: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeRecord>, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeAdjoint>, kBlock, 0); break;
     }
     if (e != hipSuccess || nb <= 0) return 1;
@@ -490,12 +557,13 @@ int render_blocks_per_cu(int mode) {
 
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1) {
-    k_fill_u32<<<1, 64, 0, s>>>(W.counter, 1, 0u);
+    k_fill_u32<<<1, 64 * kCounterStride / 4, 0, s>>>(W.counter, W.nbands * kCounterStride, 0u);
     if (ev0) (void)hipEventRecord(ev0, s);
     switch (mode) {
     case kModeForward: k_render<kModeForward><<<grid, kBlock, 0, s>>>(P, W); break;
     case kModeForwardSq: k_render<kModeForwardSq><<<grid, kBlock, 0, s>>>(P, W); break;
     case kModeCount: k_render<kModeCount><<<grid, kBlock, 0, s>>>(P, W); break;
+    case kModeRecord: k_render<kModeRecord><<<grid, kBlock, 0, s>>>(P, W); break;
     default: k_render<kModeAdjoint><<<grid, kBlock, 0, s>>>(P, W); break;
     }
     hipError_t e = hipGetLastError();
@@ -508,6 +576,21 @@ hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t gr
         e = hipGetLastError();
     }
     return e;
+}
+
+__global__ void k_grad_sum(const float *part, uint32_t copies, size_t n, float *out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float acc = out[i];
+        for (uint32_t c = 0; c < copies; ++c) acc += part[(size_t)c * n + i];
+        out[i] = acc;
+    }
+}
+
+hipError_t launch_grad_sum(const float *part, uint32_t copies, size_t n, float *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t nb = std::min<size_t>((n + 255) / 256, 65536);
+    k_grad_sum<<<(uint32_t)nb, 256, 0, s>>>(part, copies, n, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {

@@ -118,6 +118,38 @@ class RenderKernel:
                                                   int(seed), int(sample_base), C.c_void_p(adj_l.data_ptr()),
                                                   C.c_void_p(adj_e.data_ptr()), C.c_void_p(grad.data_ptr()), s))
 
+    def record_bytes(self, x0, y0, tilex, tiley, samples):
+        """Size of the per-sample path record of one render call."""
+        n = C.c_size_t()
+        L.check(L.lib().drk_kernel_record_bytes(self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples),
+                                                C.byref(n)))
+        return int(n.value)
+
+    def render_record_device(self, x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, rec,
+                             stream=None):
+        """render_device that also writes the path record into rec (a CUDA
+        tensor of at least record_bytes() bytes) for adjoint_record_device."""
+        for t in (out_l, out_e):
+            assert t.is_cuda and str(t.dtype) == "torch.float32" and t.is_contiguous() and t.numel() >= tilex * tiley * 4
+        assert rec.is_cuda and rec.is_contiguous()
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_kernel_render_record_device(
+            self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples), int(seed), int(sample_base),
+            C.c_void_p(out_l.data_ptr()), C.c_void_p(out_e.data_ptr()), C.c_void_p(rec.data_ptr()),
+            rec.numel() * rec.element_size(), s))
+
+    def adjoint_record_device(self, x0, y0, tilex, tiley, samples, seed, sample_base, adj_l, adj_e, grad, rec,
+                              stream=None):
+        """adjoint_device from the path record of the same render call."""
+        for t in (adj_l, adj_e, grad):
+            assert t.is_cuda and t.dtype.is_floating_point and t.is_contiguous()
+        assert rec.is_cuda and rec.is_contiguous()
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_kernel_adjoint_record_device(
+            self._h, int(x0), int(y0), int(tilex), int(tiley), int(samples), int(seed), int(sample_base),
+            C.c_void_p(adj_l.data_ptr()), C.c_void_p(adj_e.data_ptr()), C.c_void_p(grad.data_ptr()),
+            C.c_void_p(rec.data_ptr()), rec.numel() * rec.element_size(), s))
+
     def count(self, x0, y0, tilex, tiley, samples, seed, sample_base=0):
         """Node-record reads and brick lookups of one render (counting pass)."""
         c = (C.c_ulonglong * 2)()

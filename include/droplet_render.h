@@ -174,7 +174,7 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
  * floats, the layout of Scene::pvol[FOG]) for the render with the same tile,
  * samples, seed and sample_base. The majorants (qval) are held fixed. The
  * estimator is path replay with differential delta tracking: unbiased for the
- * derivative of the expected image, no stored paths.
+ * derivative of the expected image, no stored path vertices.
  * _device: device pointers, accumulates (the caller zeroes grad), async on
  * stream. Host form: host pointers, grad_fog overwritten, synchronous.
  * Not supported with depth compositing or scattevs > 64.
@@ -185,6 +185,28 @@ int drk_kernel_adjoint_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t 
 int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, const float *adj_lights, const float *adj_env,
                        float *grad_fog);
+
+/*
+ * Path record (no reference counterpart): the forward render keeps, per
+ * (pixel, sample) of the rendered extent, the path's lights and environment
+ * radiance, its alpha and which of its shadow rays escaped (40 bytes each;
+ * drk_kernel_record_bytes gives the size of a call's record). The adjoint of
+ * the same call (same tile, samples, seed, sample_base and scene densities)
+ * given that record skips the replay that recomputes those totals, so it
+ * traces each path once instead of twice; the gradient is the same
+ * (identical per-path contributions, summed in another order). d_record is a
+ * 16-byte aligned device buffer of at least record_bytes bytes. Not supported
+ * with scattevs > 64.
+ */
+int drk_kernel_record_bytes(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                            uint32_t samples, size_t *bytes);
+int drk_kernel_render_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                    uint32_t samples, uint64_t seed, uint32_t sample_base, float *d_lights,
+                                    float *d_env, void *d_record, size_t record_bytes, void *stream);
+int drk_kernel_adjoint_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                     uint32_t samples, uint64_t seed, uint32_t sample_base,
+                                     const float *d_adj_lights, const float *d_adj_env, float *d_grad_fog,
+                                     const void *d_record, size_t record_bytes, void *stream);
 
 /*
  * Extensions with no reference counterpart.

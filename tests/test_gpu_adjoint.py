@@ -123,5 +123,47 @@ def test_autograd_function(pkg, gpu):
         step.clamp_(min=0.0)
         il2, ie2 = AG.render(k, step.contiguous(), samples=wl.spp, seed=1)
         loss2 = ((ie2[..., :3] / wl.spp - target_e[..., :3]) ** 2).sum()
-    assert float(loss2) < float(loss)
+    assert float(loss2) < float(loss.detach())
+    k.Destroy()
+
+
+@pytest.mark.parametrize("scattevs", [3, 40])
+def test_record_adjoint_matches_replay(pkg, gpu, scattevs):
+    """Forward with the path record writes the same sums as the plain forward,
+    and the adjoint from the record equals the two-pass replay (same per-path
+    terms; only the order of the float atomics differs). scattevs 40 uses the
+    high word of the shadow-escape bits."""
+    import torch
+    from droplet_render_amd import _lib as L
+    wl = pkg.configs.tiny(n=32, w=40, h=36, spp=12, scattevs=scattevs)
+    wl.scene = raised_scene(pkg, wl)
+    k = wl.kernel()
+    x0, y0, tx, ty = 16, 10, 30, 28        # a sub-tile clipped by the image edge to 24 x 26
+    n = tx * ty * 4
+    ref_l, ref_e = torch.empty(n, device="cuda"), torch.empty(n, device="cuda")
+    out_l, out_e = torch.empty(n, device="cuda"), torch.empty(n, device="cuda")
+    nb = k.record_bytes(x0, y0, tx, ty, wl.spp)
+    assert nb == 24 * 26 * wl.spp * 40
+    rec = torch.empty(nb // 4, dtype=torch.float32, device="cuda")
+    k.render_device(x0, y0, tx, ty, wl.spp, 8, 3, ref_l, ref_e)
+    k.render_record_device(x0, y0, tx, ty, wl.spp, 8, 3, out_l, out_e, rec)
+    torch.cuda.synchronize()
+    assert torch.equal(out_l, ref_l) and torch.equal(out_e, ref_e)
+    rng = np.random.default_rng(4)
+    gl = torch.tensor(rng.normal(size=n).astype(np.float32) * 100.0, device="cuda")
+    ge = torch.tensor(rng.normal(size=n).astype(np.float32), device="cuda")
+    g_rep = torch.zeros(wl.scene.leafx[1] * wl.scene.lvoxc3, device="cuda")
+    g_rec = torch.zeros_like(g_rep)
+    k.adjoint_device(x0, y0, tx, ty, wl.spp, 8, 3, gl, ge, g_rep)
+    k.adjoint_record_device(x0, y0, tx, ty, wl.spp, 8, 3, gl, ge, g_rec, rec)
+    torch.cuda.synchronize()
+    a, b = g_rep.double().cpu().numpy(), g_rec.double().cpu().numpy()
+    assert np.isfinite(b).all() and np.linalg.norm(a) > 0
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a), np.linalg.norm(a - b) / np.linalg.norm(a)
+    assert np.array_equal(a == 0, b == 0)
+    # A record shorter than the call needs is rejected before any launch.
+    with pytest.raises(L.DrkError):
+        k.adjoint_record_device(x0, y0, tx, ty, wl.spp + 1, 8, 3, gl, ge, g_rec, rec)
+    with pytest.raises(L.DrkError):
+        k.render_record_device(x0, y0, tx, ty, wl.spp + 1, 8, 3, out_l, out_e, rec)
     k.Destroy()

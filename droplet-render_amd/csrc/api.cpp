@@ -76,6 +76,12 @@ struct drk_kernel {
     uint32_t grad_copies = 1;       // gradient copies of the adjoint (one per XCD by default)
     float *d_gradx = nullptr;       // grad_copies * (FOG gradient size) floats
     size_t gradx_bytes = 0;
+    // Adjoint gradient events: budget (bytes; 0: direct atomics only), slots
+    // allocated, and the buffers (events, sorted copy, bin tables).
+    size_t ev_budget = SIZE_MAX;    // SIZE_MAX: a quarter of device memory
+    uint32_t ev_cap = 0;
+    void *d_ev = nullptr;
+    uint32_t *d_evbins = nullptr;   // count (2 words) + hist + bin_start + bin_fill + blk_start
     float *d_depth = nullptr;
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
@@ -460,6 +466,49 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     W.rec = rec;
     uint32_t copies = 1;
     const size_t gn = (size_t)k->scene->leafx[1] * k->P.lvoxc3;
+    EventReduce E;
+    std::memset(&E, 0, sizeof(E));
+    if (adj) {
+        // Event buffer: 48 bytes per slot (event + sorted copy), sized to 16
+        // events per sample of this call within the budget (a multiple of 64
+        // slots, which keeps every plane 16-byte aligned); kept for later
+        // calls and reallocated when a call wants more.
+        uint32_t shift = 0, nbins = 0;
+        const bool binnable = event_bins(k->scene->leafx[1], k->P.lvoxc3, shift, nbins);
+        size_t budget = k->ev_budget;
+        if (budget == SIZE_MAX) {
+            size_t fr = 0, tot = 0;
+            budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : 0;
+        }
+        const uint64_t want = std::min<uint64_t>((uint64_t)npix * samples * 16, 0xffffffffull);
+        const uint64_t cap = binnable ? (std::min<uint64_t>(want, budget / kEventSlotBytes) & ~63ull) : 0;
+        if (cap > k->ev_cap) {
+            if (k->d_ev) (void)hipFree(k->d_ev);
+            if (k->d_evbins) (void)hipFree(k->d_evbins);
+            k->d_ev = nullptr;
+            k->d_evbins = nullptr;
+            k->ev_cap = 0;
+            HIP_TRY(hipMalloc(&k->d_ev, (size_t)cap * kEventSlotBytes), "adjoint: event buffer");
+            HIP_TRY(hipMalloc(&k->d_evbins, (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t)), "adjoint: event bins");
+            k->ev_cap = (uint32_t)cap;
+        }
+        if (cap > 0) {
+            const uint32_t c = k->ev_cap;
+            W.ev_base = static_cast<char *>(k->d_ev);
+            W.ev_count = reinterpret_cast<unsigned long long *>(k->d_evbins);
+            W.ev_cap = c;
+            E.ev_base = W.ev_base;
+            E.bins = k->d_evbins;
+            E.ev_cap = c;
+            E.nbins = nbins;
+            E.shift = shift;
+            E.nbricks = k->scene->leafx[1];
+            E.lvoxc = k->P.lvoxc;
+            E.lvoxc3 = k->P.lvoxc3;
+            E.grad = adj->grad;
+            HIP_TRY(launch_fill(reinterpret_cast<float *>(k->d_evbins), 2, 0.0f, s), "adjoint: clear");
+        }
+    }
     if (adj) {
         W.adj_l = adj->adj_l;
         W.adj_e = adj->adj_e;
@@ -508,6 +557,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     }
     HIP_TRY(launch_render(P, W, grid, mode, s, e0, e1), "render: launch");
     if (adj && copies > 1) HIP_TRY(launch_grad_sum(k->d_gradx, copies, gn, adj->grad, s), "adjoint: gradient sum");
+    if (adj && E.ev_cap) HIP_TRY(launch_event_reduce(E, s), "adjoint: event reduction");
     return DRK_OK;
 }
 
@@ -661,6 +711,17 @@ int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
     return DRK_OK;
 }
 
+int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_event_budget: NULL kernel");
+    k->ev_budget = bytes == UINT64_MAX ? SIZE_MAX : (size_t)bytes;
+    if (k->d_ev) (void)hipFree(k->d_ev);
+    if (k->d_evbins) (void)hipFree(k->d_evbins);
+    k->d_ev = nullptr;
+    k->d_evbins = nullptr;
+    k->ev_cap = 0;
+    return DRK_OK;
+}
+
 int drk_kernel_set_timing(drk_kernel *k, int enable) {
     if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_timing: NULL kernel");
     k->timing = enable != 0;
@@ -701,6 +762,8 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_env) (void)hipFree(k->d_env);
     if (k->d_kc) (void)hipFree(k->d_kc);
     if (k->d_gradx) (void)hipFree(k->d_gradx);
+    if (k->d_ev) (void)hipFree(k->d_ev);
+    if (k->d_evbins) (void)hipFree(k->d_evbins);
     if (k->d_depth) (void)hipFree(k->d_depth);
     for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
     for (auto &pr : k->ev_free) {

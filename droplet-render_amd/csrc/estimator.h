@@ -248,18 +248,21 @@ __device__ __forceinline__ float brick_lookup(const float *__restrict__ brick, F
     return nl[2] * (v1 - v0) + v0;
 }
 
-// Derivative of brick_lookup with respect to the 8 corner values it reads:
-// corner index (z*l^2 + y*l + x) and weight, with exactly brick_lookup's index
-// and fraction arithmetic (d lerp(a,b,t)/da = 1-t, /db = t); clamped corners
+// Derivative of brick_lookup with respect to the 8 corner values it reads,
+// with exactly brick_lookup's index and fraction arithmetic; clamped corners
 // may repeat, their weights then add up.
-__device__ __forceinline__ void brick_weights(F3 p, float4 ce, uint32_t lvoxc, uint32_t idx[8], float w[8]) {
+// Cell of the trilinear lookup at p: lower corner ia (per axis), whether the
+// upper corner differs (d = ib - ia, 0 or 1 after the clamps) and the lerp
+// fractions nl (not clamped, as in the lookup). Packed as
+// ia.x | ia.y << 8 | ia.z << 16 | d.x << 24 | d.y << 25 | d.z << 26
+// (lvoxc <= 256).
+__device__ __forceinline__ uint32_t brick_cell(F3 p, float4 ce, uint32_t lvoxc, float nl[3]) {
     const float nv = (float)lvoxc;
     const float nvm = nv - 1.0f;
     float ni[3] = {((-0.5f * ((ce.x - ce.w) - p.x)) * nvm) / ce.w,
                    ((-0.5f * ((ce.y - ce.w) - p.y)) * nvm) / ce.w,
                    ((-0.5f * ((ce.z - ce.w) - p.z)) * nvm) / ce.w};
-    uint32_t ia[3], ib[3];
-    float nl[3];
+    uint32_t cell = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         float nf = fmaxf(floorf(ni[k]), 0.0f);
@@ -267,16 +270,23 @@ __device__ __forceinline__ void brick_weights(F3 p, float4 ce, uint32_t lvoxc, u
         nl[k] = ni[k] - nf;
         nc = fmaxf(fminf(nc, nvm), 0.0f);
         nf = fmaxf(fminf(nf, nvm), 0.0f);
-        ia[k] = (uint32_t)nf;
-        ib[k] = (uint32_t)nc;
+        const uint32_t ia = (uint32_t)nf, ib = (uint32_t)nc;
+        cell |= (ia << (8 * k)) | ((ib - ia) << (24 + k));
     }
-    const uint32_t l2 = lvoxc * lvoxc;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        const bool bx = v & 1, by = v & 2, bz = v & 4;
-        idx[v] = (bz ? ib[2] : ia[2]) * l2 + (by ? ib[1] : ia[1]) * lvoxc + (bx ? ib[0] : ia[0]);
-        w[v] = (bx ? nl[0] : 1.0f - nl[0]) * (by ? nl[1] : 1.0f - nl[1]) * (bz ? nl[2] : 1.0f - nl[2]);
-    }
+    return cell;
+}
+
+// Corner v (bit 0: x, 1: y, 2: z) of a packed cell: its index in the brick
+// and its trilinear weight, d(lookup)/d(corner value) (lerp(a,b,t) has
+// derivatives 1-t and t).
+__device__ __forceinline__ void cell_corner(uint32_t cell, const float nl[3], uint32_t lvoxc, int v, uint32_t &idx,
+                                            float &w) {
+    const uint32_t bx = v & 1, by = (v >> 1) & 1, bz = (v >> 2) & 1;
+    const uint32_t x = (cell & 255u) + (bx & (cell >> 24));
+    const uint32_t y = ((cell >> 8) & 255u) + (by & (cell >> 25));
+    const uint32_t z = ((cell >> 16) & 255u) + (bz & (cell >> 26));
+    idx = z * (lvoxc * lvoxc) + y * lvoxc + x;
+    w = (bx ? nl[0] : 1.0f - nl[0]) * (by ? nl[1] : 1.0f - nl[1]) * (bz ? nl[2] : 1.0f - nl[2]);
 }
 
 // ---------------------------------------------------------------------------

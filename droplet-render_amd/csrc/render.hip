@@ -72,13 +72,18 @@ __device__ __forceinline__ uint32_t fetch_units(uint32_t *counter, bool need) {
 }
 
 template <int kMode>
-__global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W) {
+__device__ __forceinline__ void render_body(const RenderParams &P, const WorkParams &W) {
     constexpr bool kSumSq = (kMode == kModeForwardSq || kMode == kModeCount);
     constexpr bool kCount = (kMode == kModeCount);
     constexpr bool kAdj = (kMode == kModeAdjoint);
     constexpr bool kRec = (kMode == kModeRecord);
     __shared__ MieLds mie;
     __shared__ uint32_t stk_all[kMaxDepth * kBlock];
+    // Adjoint: per wave, the next free slot and the end of the chunk of the
+    // event buffer it has reserved (in LDS, so that every lane sees the
+    // updates made under divergent control flow).
+    __shared__ unsigned long long ev_cur[kBlock / 64][2];
+    if (kAdj && (threadIdx.x & 63) == 0) ev_cur[threadIdx.x >> 6][0] = ev_cur[threadIdx.x >> 6][1] = 0;
     load_mie(mie);
     __syncthreads();
 
@@ -113,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
     float4 ce = make_float4(0, 0, 0, 1);
     const float *brick = nullptr;
     size_t brick_off = 0;
+    uint32_t leaf_brick = 0;
     float smax = 1.0f, kmaj = 1.0f, lo = 0.0f, s0 = 0.0f, tr0 = 0.0f, tr1 = 0.0f, trb = 0.0f, sr = 0.0f, td = 0.0f;
     bool first_step = true;
     uint32_t cnt_nodes = 0, cnt_lookups = 0;
@@ -206,15 +212,57 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
     };
 
     // Adds weight * (trilinear weight of vertex v at x) to every corner v of
-    // the current leaf's gradient brick (adjoint pass 1 only).
+    // the current leaf's gradient brick (adjoint pass 1 only): as one event
+    // {weight, cell} in the event buffer (one 64-bit atomic per wave for the
+    // slots), or, once the buffer is full, as 8 float atomics into the brick.
     auto grad_at = [&](F3 x, float weight) {
         if (weight == 0.0f) return;
-        uint32_t idx[8];
-        float w8[8];
-        brick_weights(x, ce, P.lvoxc, idx, w8);
-        float *g = grad_copy + brick_off;
+        float nl[3];
+        const uint32_t cell = brick_cell(x, ce, P.lvoxc, nl);
+        bool queued = false;
+        if (W.ev_cap) {
+            // Slots come from the wave's reserved chunk; when it runs out, the
+            // wave's first active lane reserves the next kEvChunk slots with
+            // one atomic (a single counter updated per event would serialise).
+            const uint64_t m = __ballot(true);
+            const uint32_t lane = lane_id();
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            unsigned long long *cur = ev_cur[threadIdx.x >> 6];
+            const unsigned long long c0 = cur[0], avail = cur[1] - c0;
+            unsigned long long fresh = 0;
+            if (cnt > avail) {
+                if (lane == leader) fresh = atomicAdd(W.ev_count, (unsigned long long)kEvChunk);
+                const uint32_t lo = __shfl((uint32_t)fresh, (int)leader), hi = __shfl((uint32_t)(fresh >> 32), (int)leader);
+                fresh = ((unsigned long long)hi << 32) | lo;
+            }
+            const unsigned long long slot = rank < avail ? c0 + rank : fresh + (rank - avail);
+            if (lane == leader) {
+                if (cnt > avail) {
+                    cur[0] = fresh + (cnt - avail);
+                    cur[1] = fresh + kEvChunk;
+                } else {
+                    cur[0] = c0 + cnt;
+                }
+            }
+            if (slot < W.ev_cap) {
+                const EventPlanes ev = event_planes(W.ev_base, W.ev_cap);
+                ev.wf[slot] = make_float4(weight, nl[0], nl[1], nl[2]);
+                ev.bc[slot] = make_uint2(leaf_brick, cell);
+                queued = true;
+            }
+        }
+        if (!queued) {
+            float *g = grad_copy + brick_off;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) unsafeAtomicAdd(g + idx[v], weight * w8[v]);
+            for (int v = 0; v < 8; ++v) {
+                uint32_t idx;
+                float w;
+                cell_corner(cell, nl, P.lvoxc, v, idx, w);
+                unsafeAtomicAdd(g + idx, weight * w);
+            }
+        }
     };
     // -W * sigma_e * integral over [a, b] of the trilinear weights, by one
     // uniform point of the adjoint stream (drawn for every covered leaf with a
@@ -302,7 +350,8 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
                 const drk_octree_node &n = P.ob[leaf];
                 ce = *reinterpret_cast<const float4 *>(n.ce);
                 smax = n.qval[DRK_VOLUME_FOG];
-                brick_off = (size_t)n.volx[DRK_VOLUME_FOG] * P.lvoxc3;
+                leaf_brick = n.volx[DRK_VOLUME_FOG];
+                brick_off = (size_t)leaf_brick * P.lvoxc3;
                 brick = P.pvol_fog + brick_off;
                 if (kCount) cnt_nodes += 1;
                 // Fog-only leaf: always skip to the leaf entry (src/kernel.cpp:233-269).
@@ -501,10 +550,32 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W)
             }
         }
     }
+    if (kAdj && W.ev_cap) {
+        // The unused tail of the wave's last chunk gets zero-weight events.
+        const unsigned long long *cur = ev_cur[threadIdx.x >> 6];
+        const unsigned long long end = min(cur[1], (unsigned long long)W.ev_cap);
+        const EventPlanes ev = event_planes(W.ev_base, W.ev_cap);
+        for (unsigned long long i = cur[0] + lane_id(); i < end; i += 64) {
+            ev.wf[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ev.bc[i] = make_uint2(0u, 0u);
+        }
+    }
     if (kCount) {
         atomicAdd(reinterpret_cast<unsigned long long *>(W.counts) + 0, (unsigned long long)cnt_nodes);
         atomicAdd(reinterpret_cast<unsigned long long *>(W.counts) + 1, (unsigned long long)cnt_lookups);
     }
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W) {
+    render_body<kMode>(P, W);
+}
+
+// The adjoint carries more live state: held to 168 VGPRs so that three
+// waves per SIMD stay resident, as in the forward instantiations.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render_adjoint(RenderParams P,
+                                                                                                    WorkParams W) {
+    render_body<kModeAdjoint>(P, W);
 }
 
 // Sums the per-chunk partials of each pixel in chunk order (deterministic).
@@ -549,7 +620,7 @@ int render_blocks_per_cu(int mode) {
     case kModeForwardSq: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeForwardSq>, kBlock, 0); break;
     case kModeCount: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeCount>, kBlock, 0); break;
     case kModeRecord: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeRecord>, kBlock, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeAdjoint>, kBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_adjoint, kBlock, 0); break;
     }
     if (e != hipSuccess || nb <= 0) return 1;
     return nb;
@@ -564,7 +635,7 @@ hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t gr
     case kModeForwardSq: k_render<kModeForwardSq><<<grid, kBlock, 0, s>>>(P, W); break;
     case kModeCount: k_render<kModeCount><<<grid, kBlock, 0, s>>>(P, W); break;
     case kModeRecord: k_render<kModeRecord><<<grid, kBlock, 0, s>>>(P, W); break;
-    default: k_render<kModeAdjoint><<<grid, kBlock, 0, s>>>(P, W); break;
+    default: k_render_adjoint<<<grid, kBlock, 0, s>>>(P, W); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -590,6 +661,177 @@ hipError_t launch_grad_sum(const float *part, uint32_t copies, size_t n, float *
     if (n == 0) return hipSuccess;
     size_t nb = std::min<size_t>((n + 255) / 256, 65536);
     k_grad_sum<<<(uint32_t)nb, 256, 0, s>>>(part, copies, n, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Gradient events: counting sort by bin, then LDS accumulation per bin.
+// Every kernel reads the event count from device memory and clamps it to the
+// capacity, so the launches are sized by capacity and need no host sync.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kEvSortChunk = 65536;   // events per sort block
+constexpr uint32_t kEvSortBlock = 1024;
+constexpr uint32_t kEvAccChunk = 16384;    // events per accumulation block
+constexpr uint32_t kEvAccBlock = 256;
+
+__device__ __forceinline__ uint32_t ev_n(const EventReduce &E) {
+    const unsigned long long c = *reinterpret_cast<const unsigned long long *>(E.bins);
+    return c < E.ev_cap ? (uint32_t)c : E.ev_cap;
+}
+
+// Bin tables after the 2-word count.
+struct BinTables {
+    uint32_t *hist, *bin_start, *bin_fill, *blk_start;
+};
+__device__ __forceinline__ BinTables bin_tables(const EventReduce &E) {
+    BinTables t;
+    t.hist = E.bins + 2;
+    t.bin_start = t.hist + E.nbins;
+    t.bin_fill = t.bin_start + E.nbins + 1;
+    t.blk_start = t.bin_fill + E.nbins;
+    return t;
+}
+
+__global__ __launch_bounds__(kEvSortBlock) void k_ev_hist(EventReduce E) {
+    extern __shared__ uint32_t lh[];
+    const uint32_t n = ev_n(E);
+    const uint64_t start = (uint64_t)blockIdx.x * kEvSortChunk;
+    if (start >= n) return;
+    const uint32_t end = (uint32_t)min<uint64_t>(start + kEvSortChunk, n);
+    const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
+    const BinTables t = bin_tables(E);
+    for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) lh[b] = 0;
+    __syncthreads();
+    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.bc[i].x >> E.shift], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock)
+        if (lh[b]) atomicAdd(&t.hist[b], lh[b]);
+}
+
+// Exclusive scans of the bin sizes (event offsets) and of their chunk counts
+// (accumulation block offsets); resets the scatter cursors.
+__global__ __launch_bounds__(1024) void k_ev_scan(EventReduce E) {
+    __shared__ uint32_t part_e[1024], part_b[1024];
+    const BinTables T = bin_tables(E);
+    const uint32_t per = (E.nbins + 1023) / 1024;
+    const uint32_t b0 = min(threadIdx.x * per, E.nbins), b1 = min(b0 + per, E.nbins);
+    uint32_t se = 0, sb = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        se += T.hist[b];
+        sb += (T.hist[b] + kEvAccChunk - 1) / kEvAccChunk;
+    }
+    part_e[threadIdx.x] = se;
+    part_b[threadIdx.x] = sb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ae = 0, ab = 0;
+        for (uint32_t t = 0; t < 1024; ++t) {
+            const uint32_t e = part_e[t], b = part_b[t];
+            part_e[t] = ae;
+            part_b[t] = ab;
+            ae += e;
+            ab += b;
+        }
+        T.bin_start[E.nbins] = ae;
+        T.blk_start[E.nbins] = ab;
+    }
+    __syncthreads();
+    se = part_e[threadIdx.x];
+    sb = part_b[threadIdx.x];
+    for (uint32_t b = b0; b < b1; ++b) {
+        T.bin_start[b] = se;
+        T.blk_start[b] = sb;
+        T.bin_fill[b] = 0;
+        se += T.hist[b];
+        sb += (T.hist[b] + kEvAccChunk - 1) / kEvAccChunk;
+    }
+}
+
+__global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
+    extern __shared__ uint32_t lds[];
+    uint32_t *lh = lds, *lb = lds + E.nbins;
+    const uint32_t n = ev_n(E);
+    const uint64_t start = (uint64_t)blockIdx.x * kEvSortChunk;
+    if (start >= n) return;
+    const uint32_t end = (uint32_t)min<uint64_t>(start + kEvSortChunk, n);
+    const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
+    const BinTables t = bin_tables(E);
+    for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) lh[b] = 0;
+    __syncthreads();
+    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.bc[i].x >> E.shift], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) {
+        if (lh[b]) lb[b] = t.bin_start[b] + atomicAdd(&t.bin_fill[b], lh[b]);
+        lh[b] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) {
+        const uint2 bc = ev.bc[i];
+        const uint32_t b = bc.x >> E.shift;
+        const uint32_t pos = lb[b] + atomicAdd(&lh[b], 1u);
+        ev.swf[pos] = ev.wf[i];
+        ev.sbc[pos] = bc;
+    }
+}
+
+__global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
+    extern __shared__ float lg[];
+    const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
+    const BinTables t = bin_tables(E);
+    const uint32_t total = t.blk_start[E.nbins];
+    if (blockIdx.x >= total) return;
+    // Bin of this block: the last b with blk_start[b] <= blockIdx.x.
+    uint32_t lo = 0, hi = E.nbins;   // blk_start[lo] <= blockIdx.x < blk_start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (t.blk_start[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t b = lo;
+    const uint32_t e0 = t.bin_start[b] + (blockIdx.x - t.blk_start[b]) * kEvAccChunk;
+    const uint32_t e1 = min(e0 + kEvAccChunk, t.bin_start[b + 1]);
+    const uint32_t brick0 = b << E.shift;
+    const uint32_t span = min(1u << E.shift, E.nbricks - brick0) * E.lvoxc3;
+    for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lg[j] = 0.0f;
+    __syncthreads();
+    for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
+        const float4 wf = ev.swf[i];
+        const uint2 bc = ev.sbc[i];
+        const float nl[3] = {wf.y, wf.z, wf.w};
+        float *g = lg + (bc.x - brick0) * E.lvoxc3;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            uint32_t idx;
+            float w;
+            cell_corner(bc.y, nl, E.lvoxc, v, idx, w);
+            atomicAdd(g + idx, wf.x * w);
+        }
+    }
+    __syncthreads();
+    float *out = E.grad + (size_t)brick0 * E.lvoxc3;
+    for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) {
+        const float v = lg[j];
+        if (v != 0.0f) unsafeAtomicAdd(out + j, v);
+    }
+}
+
+bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nbins) {
+    shift = 0;
+    while (((uint64_t)nbricks + (1ull << shift) - 1) >> shift > kEvMaxBins) ++shift;
+    nbins = (uint32_t)(((uint64_t)nbricks + (1ull << shift) - 1) >> shift);
+    return nbricks > 0 && ((uint64_t)lvoxc3 * sizeof(float) << shift) <= kEvAccumLds;
+}
+
+hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
+    if (E.ev_cap == 0 || E.nbins == 0) return hipSuccess;
+    const uint32_t sort_blocks = (uint32_t)(((uint64_t)E.ev_cap + kEvSortChunk - 1) / kEvSortChunk);
+    const uint32_t acc_blocks = (uint32_t)(((uint64_t)E.ev_cap + kEvAccChunk - 1) / kEvAccChunk) + E.nbins;
+    k_fill_u32<<<(E.nbins + 255) / 256, 256, 0, s>>>(E.bins + 2, E.nbins, 0u);
+    k_ev_hist<<<sort_blocks, kEvSortBlock, E.nbins * sizeof(uint32_t), s>>>(E);
+    k_ev_scan<<<1, 1024, 0, s>>>(E);
+    k_ev_scatter<<<sort_blocks, kEvSortBlock, 2 * E.nbins * sizeof(uint32_t), s>>>(E);
+    const size_t lds = (size_t)E.lvoxc3 * sizeof(float) << E.shift;
+    k_ev_accum<<<acc_blocks, kEvAccBlock, lds, s>>>(E);
     return hipGetLastError();
 }
 

@@ -26,6 +26,13 @@ struct WorkParams {
     unsigned long long *counts;  // [node touches, brick lookups] for the counting pass
     const float *adj_l, *adj_e;  // adjoint mode: dL/d(phb sums), tile float4 (stride tilex)
     float *grad;                 // adjoint mode: dL/d(FOG brick values), accumulated
+    // Adjoint gradient events (binned by brick and reduced in LDS after the
+    // launch; once ev_cap is reached, lanes add to the bricks directly).
+    // Both buffers are passed by their allocation base (the planes are
+    // located in the kernels; see EventPlanes).
+    unsigned long long *ev_count;  // events produced (may exceed ev_cap); base of the bin tables
+    char *ev_base;                 // event buffer
+    uint32_t ev_cap;               // event slots; 0: direct atomics only
     size_t grad_stride;          // floats between gradient copies
     uint32_t grad_copies;        // copies the waves spread their atomics over (>= 1)
     uint32_t grad_nxcc;          // XCDs of the device
@@ -54,5 +61,43 @@ hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t gr
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 // out[i] += sum over c < copies of part[c * n + i].
 hipError_t launch_grad_sum(const float *part, uint32_t copies, size_t n, float *out, hipStream_t s);
+
+// Reduction of the adjoint's gradient events into the FOG gradient: a
+// counting sort by bin (2^shift bricks per bin) followed by one LDS
+// accumulation per (bin, chunk of events).
+struct EventReduce {
+    char *ev_base;         // event buffer (see EventPlanes)
+    uint32_t *bins;        // bin tables: count (2 words), hist[nbins], bin_start[nbins+1],
+                           // bin_fill[nbins], blk_start[nbins+1]
+    uint32_t ev_cap;
+    uint32_t nbins, shift; // bin = brick >> shift
+    uint32_t nbricks, lvoxc, lvoxc3;
+    float *grad;           // accumulated into
+};
+
+// Planes of the event buffer (cap slots each): events {weight, fractions}
+// float4 and {brick, cell} uint2, then their copies sorted by bin.
+struct EventPlanes {
+    float4 *wf;
+    uint2 *bc;
+    float4 *swf;
+    uint2 *sbc;
+};
+__host__ __device__ inline EventPlanes event_planes(char *base, uint32_t cap) {
+    EventPlanes e;
+    e.wf = reinterpret_cast<float4 *>(base);
+    e.bc = reinterpret_cast<uint2 *>(base + (size_t)cap * 16);
+    e.swf = reinterpret_cast<float4 *>(base + (size_t)cap * 24);
+    e.sbc = reinterpret_cast<uint2 *>(base + (size_t)cap * 40);
+    return e;
+}
+constexpr uint32_t kEvMaxBins = 8192;          // LDS histogram of the sort
+constexpr size_t kEventSlotBytes = 48;         // event (24 B) + its sorted copy (24 B)
+constexpr uint32_t kEvChunk = 2048;            // event slots a wave reserves at a time
+constexpr uint32_t kEvAccumLds = 65536;        // bytes of LDS gradient per accumulation block
+// Largest shift (bricks per bin = 2^shift) and bin count for a scene, or
+// false when a bin's bricks do not fit the accumulation block's LDS.
+bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nbins);
+hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s);
 
 }  // namespace drk

@@ -118,6 +118,11 @@ class RenderKernel:
                                                   int(seed), int(sample_base), C.c_void_p(adj_l.data_ptr()),
                                                   C.c_void_p(adj_e.data_ptr()), C.c_void_p(grad.data_ptr()), s))
 
+    def set_event_budget(self, nbytes=None):
+        """Limit of the adjoint's gradient-event buffer in bytes (None: the
+        default, a quarter of device memory; 0: float atomics only)."""
+        L.check(L.lib().drk_kernel_set_event_budget(self._h, (1 << 64) - 1 if nbytes is None else int(nbytes)))
+
     def record_bytes(self, x0, y0, tilex, tiley, samples):
         """Size of the per-sample path record of one render call."""
         n = C.c_size_t()

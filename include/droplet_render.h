@@ -198,6 +198,18 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
  * 16-byte aligned device buffer of at least record_bytes bytes. Not supported
  * with scattevs > 64.
  */
+/*
+ * Gradient events (no reference counterpart). The adjoint writes each
+ * gradient contribution as a 24-byte event, sorts the events by brick and
+ * sums them in on-chip memory, which avoids per-corner float atomics in HBM.
+ * The event buffer (48 bytes per slot, 16 slots per pixel sample of a call)
+ * is limited to `bytes` (UINT64_MAX: a quarter of device memory, the
+ * default; 0: no events, every contribution is added with atomics); events
+ * beyond it are added with atomics. The gradient is the same either way up
+ * to the order of float additions.
+ */
+int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes);
+
 int drk_kernel_record_bytes(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
                             uint32_t samples, size_t *bytes);
 int drk_kernel_render_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,

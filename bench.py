@@ -35,6 +35,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 1 GiB device copies (a kernel of known traffic, for scaling the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -45,6 +48,16 @@ def main():
     import numpy as np
     import torch
     from __graft_entry__ import load_package
+
+    if args.copy_calibration:
+        n = 1 << 28   # floats: 1 GiB per copy
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+        return
 
     pkg = load_package()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -98,7 +111,8 @@ def main():
     # Forward + adjoint: one step = the forward render (keeping the per-sample
     # path record) plus the path-replay adjoint of the same samples for a
     # dense image gradient (every pixel weighted), accumulated into FP32
-    # gradient bricks.
+    # gradient bricks. Also timed: the adjoint without the record (it replays
+    # each path twice and keeps no per-sample state).
     fwd_adj = None
     if not args.no_adjoint:
         g = torch.Generator(device="cpu").manual_seed(5)
@@ -113,20 +127,34 @@ def main():
             k.adjoint_record_device(0, 0, wl.w, wl.h, wl.spp, args.seed, i * wl.spp, adj_l, adj_e, grad, rec,
                                     stream=stream)
 
-        step_adj(0)
-        torch.cuda.synchronize()
-        k.timing()
-        ta = time.perf_counter()
-        for i in range(args.adj_steps):
-            step_adj(args.warmup + i)
-        torch.cuda.synchronize()
-        ta = time.perf_counter() - ta
-        kms, kn = k.timing()        # forward and adjoint launches alternate
+        def step_replay(i):
+            k.render_device(0, 0, wl.w, wl.h, wl.spp, args.seed, i * wl.spp, out_l, out_e, stream=stream)
+            k.adjoint_device(0, 0, wl.w, wl.h, wl.spp, args.seed, i * wl.spp, adj_l, adj_e, grad, stream=stream)
+
+        def timed(fn, n):
+            fn(0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for i in range(n):
+                fn(args.warmup + i)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t
+
+        k.set_timing(False)
+        torch.cuda.reset_peak_memory_stats()
+        ta = timed(step_adj, args.adj_steps)
+        tr = timed(step_replay, max(1, args.adj_steps // 2)) / max(1, args.adj_steps // 2) * args.adj_steps
+        k.set_timing(True)
+        fwd_ms = wall / args.steps
         fwd_adj = {"value": round(wl.samples * args.adj_steps / ta / 1e6, 3), "unit": "Msamples/s",
                    "steps": args.adj_steps, "ms_per_step": round(ta / args.adj_steps * 1e3, 3),
-                   "ratio_to_forward": round((ta / args.adj_steps) / (wall / args.steps), 3),
-                   "kernels_ms_per_step": round(kms / args.adj_steps, 3),
+                   "ratio_to_forward": round((ta / args.adj_steps) / fwd_ms, 3),
+                   "estimator": "path replay, differential delta tracking, trilinear splat of every term; "
+                                "forward keeps the per-sample path record (40 B/sample)",
+                   "no_record_ms_per_step": round(tr / args.adj_steps * 1e3, 3),
+                   "no_record_ratio_to_forward": round((tr / args.adj_steps) / fwd_ms, 3),
                    "record_bytes": rec_bytes,
+                   "kernel_device_bytes": k.footprint(),
                    "gradient_finite": bool(torch.isfinite(grad).all().item())}
         del rec
 

@@ -711,6 +711,24 @@ int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
     return DRK_OK;
 }
 
+int drk_kernel_footprint(const drk_kernel *k, size_t *bytes) {
+    if (!k || !bytes) return set_error(DRK_EINVAL, "drk_kernel_footprint: NULL argument");
+    size_t b = (size_t)4 * k->tilex * k->tiley * 16 + k->partial_bytes + k->gradx_bytes + sizeof(KernelConsts);
+    b += kMaxBands * kCounterStride * sizeof(uint32_t) + 64;
+    if (k->d_ev) b += (size_t)k->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
+    if (k->d_env) b += (size_t)k->C.env_w * k->C.env_h * 16;
+    if (k->d_depth) b += (size_t)k->P.w * k->P.h * sizeof(float);
+    *bytes = b;
+    return DRK_OK;
+}
+
+int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
+    if (!s || !bytes) return set_error(DRK_EINVAL, "drk_scene_footprint: NULL argument");
+    *bytes = (size_t)s->node_count * (sizeof(drk_octree_node) + 8 * sizeof(uint32_t)) +
+             (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
+    return DRK_OK;
+}
+
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes) {
     if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_event_budget: NULL kernel");
     k->ev_budget = bytes == UINT64_MAX ? SIZE_MAX : (size_t)bytes;

@@ -123,6 +123,12 @@ class RenderKernel:
         default, a quarter of device memory; 0: float atomics only)."""
         L.check(L.lib().drk_kernel_set_event_budget(self._h, (1 << 64) - 1 if nbytes is None else int(nbytes)))
 
+    def footprint(self):
+        """Device memory held by this kernel, in bytes."""
+        n = C.c_size_t()
+        L.check(L.lib().drk_kernel_footprint(self._h, C.byref(n)))
+        return int(n.value)
+
     def record_bytes(self, x0, y0, tilex, tiley, samples):
         """Size of the per-sample path record of one render call."""
         n = C.c_size_t()

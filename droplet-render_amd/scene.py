@@ -69,6 +69,12 @@ class Scene:
             self._dev = h
         return self._dev
 
+    def footprint(self):
+        """Device memory of the uploaded scene (nodes, child links, bricks), in bytes."""
+        n = C.c_size_t()
+        L.check(L.lib().drk_scene_footprint(self.device(), C.byref(n)))
+        return int(n.value)
+
     def update_fog(self, pvol_fog, qval_fog=None):
         """Replaces the FOG bricks (host array or device pointer via torch) and optionally the majorants."""
         q = None if qval_fog is None else np.ascontiguousarray(qval_fog, dtype=np.float32)

@@ -210,6 +210,12 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
  */
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes);
 
+/* Device memory held by a kernel (result, partial-sum, gradient-copy and
+ * event buffers as currently allocated) and by a scene (node records, child
+ * links, FOG bricks), in bytes. No reference counterpart. */
+int drk_kernel_footprint(const drk_kernel *k, size_t *bytes);
+int drk_scene_footprint(const drk_scene *s, size_t *bytes);
+
 int drk_kernel_record_bytes(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
                             uint32_t samples, size_t *bytes);
 int drk_kernel_render_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,

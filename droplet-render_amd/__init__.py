@@ -11,7 +11,7 @@ Package layout (the directory name has a dash, so it is loaded through
     configs   the synthetic benchmark workloads
     build     in-tree hipcc build of the library
 """
-from . import _lib, camera, sampler, scene, kernel, configs, probe  # noqa: F401
+from . import _lib, camera, sampler, scene, kernel, batch, configs, probe  # noqa: F401
 from .scene import Scene  # noqa: F401
 from .kernel import RenderKernel  # noqa: F401
 from .sampler import MiePhase, HGPhase, SunLight, HosekSky, NullEnv, MapEnv  # noqa: F401

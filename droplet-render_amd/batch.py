@@ -1,0 +1,62 @@
+"""Batched forward renders: many (kernel, tile) jobs in one persistent launch.
+
+`Batch(jobs)` takes `(kernel, x0, y0, tilex, tiley, samples, seed,
+sample_base, out_lights, out_env)` tuples (outputs: CUDA float32 tensors of
+tilex*tiley*4) and wraps drk_batch_*: `launch()` enqueues the batched kernel,
+`launch_graph()` replays it as a captured hipGraph, `autotune()` picks the
+grid and the work-unit granularity by timing a sweep.
+"""
+import ctypes as C
+
+from . import _lib as L
+
+
+def _stream_ptr(stream):
+    return None if stream is None else C.c_void_p(stream.cuda_stream)
+
+
+class Batch:
+    def __init__(self, jobs):
+        arr = (L.BatchJob * len(jobs))()
+        self._keep = []
+        for i, job in enumerate(jobs):
+            k, x0, y0, tx, ty, spp, seed, base, out_l, out_e = job[:10]
+            sq_l, sq_e = (job[10], job[11]) if len(job) > 10 else (None, None)
+            for t in (out_l, out_e, sq_l, sq_e):
+                if t is not None:
+                    assert t.is_cuda and str(t.dtype) == "torch.float32" and t.is_contiguous()
+                    assert t.numel() >= tx * ty * 4
+            arr[i] = L.BatchJob(k._h, int(x0), int(y0), int(tx), int(ty), int(spp), int(seed), int(base),
+                                out_l.data_ptr(), out_e.data_ptr(), sq_l.data_ptr() if sq_l is not None else None,
+                                sq_e.data_ptr() if sq_e is not None else None)
+            self._keep.append(k)
+        h = C.c_void_p()
+        L.check(L.lib().drk_batch_create(arr, len(jobs), C.byref(h)))
+        self._h = h
+        self.jobs = list(jobs)
+
+    def set_tuning(self, grid_blocks=0, unit_scale=1.0):
+        L.check(L.lib().drk_batch_set_tuning(self._h, int(grid_blocks), float(unit_scale)))
+
+    def launch(self, stream=None):
+        L.check(L.lib().drk_batch_launch(self._h, _stream_ptr(stream)))
+
+    def launch_graph(self, stream=None):
+        L.check(L.lib().drk_batch_launch_graph(self._h, _stream_ptr(stream)))
+
+    def autotune(self, stream=None, reps=2):
+        """Times a grid x work-unit sweep, keeps the fastest; returns the table (text)."""
+        buf = C.create_string_buffer(4096)
+        L.check(L.lib().drk_batch_autotune(self._h, _stream_ptr(stream), int(reps), buf, 4096))
+        return buf.value.decode()
+
+    def destroy(self):
+        if self._h is not None:
+            L.lib().drk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

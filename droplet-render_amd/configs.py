@@ -96,3 +96,37 @@ def tiny(n=32, w=32, h=32, spp=8, scattevs=4, sigma_t=8.0, seed=3):
     """Small fBm scene for parity tests that the CPU oracle finishes in seconds."""
     g = synth_fbm(n, seed, octaves=3, freq=3.0, threshold=0.2)
     return Workload("tiny_fbm%d" % n, g, Scene.from_grid(g), w, h, spp, scattevs, sigma_t)
+
+
+def _cloud_with_occupancy(n, seed, occupancy, kind):
+    """An fBm (kind 0) or sphere-union (kind 1) cloud whose fraction of
+    non-empty voxels is `occupancy`: the field is thresholded at the matching
+    quantile and renormalised to max 1."""
+    if kind == 0:
+        f = synth_fbm(n, seed, threshold=-4.0)            # monotone in the raw fBm value
+    else:
+        f = synth_spheres(n, seed, spheres=24, band=0.5)   # ramps wide enough that few voxels saturate
+    q = float(np.quantile(f, 1.0 - occupancy))
+    if q >= float(f.max()):
+        q = float(np.nextafter(np.float32(q), np.float32(-1.0)))
+    g = np.maximum(f - q, 0.0)
+    m = float(g.max())
+    return (g / m if m > 0 else g).astype(np.float32)
+
+
+def c2(nscenes=32, seed=2, max_n=512):
+    """Batched forward workload: `nscenes` independent clouds, grid n
+    log-uniform in [32, max_n] (multiples of 8), voxel occupancy uniform in
+    [2%, 30%], alternating fBm / sphere union, images from {128^2, 256^2,
+    512^2, 1024^2}, 16-64 spp, 16 scattering events, sigma_t 16."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nscenes):
+        n = int(round(np.exp(rng.uniform(np.log(32), np.log(max_n))) / 8.0)) * 8
+        n = max(32, min(max_n, n))
+        occ = float(rng.uniform(0.02, 0.30))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.integers(16, 65))
+        g = _cloud_with_occupancy(n, 1000 + i, occ, i % 2)
+        out.append(Workload("c2_%02d_n%d" % (i, n), g, Scene.from_grid(g), res, res, spp, 16, 16.0))
+    return out

@@ -93,6 +93,27 @@ struct drk_kernel {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free, ev_used;
 };
 
+struct drk_batch {
+    std::vector<drk_batch_job> jobs;
+    std::vector<RenderParams> hP;
+    std::vector<WorkParams> hW;
+    std::vector<uint64_t> work;      // pixel samples per job
+    RenderParams *dP = nullptr;
+    WorkParams *dW = nullptr;
+    uint32_t *d_counters = nullptr;  // kMaxBands * kCounterStride words per job
+    float *d_partial = nullptr;
+    size_t partial_bytes = 0;
+    uint32_t grid = 0;               // 0: occupancy-derived
+    uint32_t grid_eff = 1;           // grid of the planned launch
+    float unit_scale = 1.0f;         // work units per lane, relative to the default of 16
+    uint32_t cu_count = 1;
+    bool sumsq = false;
+    bool planned = false;
+    hipStream_t cap_stream = nullptr;  // stream the graph is captured on
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 extern "C" {
 
 const char *drk_last_error(void) { return g_err.c_str(); }
@@ -791,6 +812,271 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->stream) (void)hipStreamDestroy(k->stream);
     std::free(k->h_phb);
     delete k;
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: many (kernel, tile) jobs in one persistent grid.
+// ---------------------------------------------------------------------------
+static void batch_drop_graph(drk_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Fills the per-job launch parameters (tile, chunking, outputs), the partial
+// buffer and the device tables for the current grid and unit scale.
+static int batch_plan(drk_batch *b) {
+    const uint32_t nj = (uint32_t)b->jobs.size();
+    const int mode = b->sumsq ? kModeForwardSq : kModeForward;
+    const uint32_t grid = b->grid ? b->grid : b->cu_count * (uint32_t)std::max(1, render_batch_blocks_per_cu(mode));
+    b->grid_eff = grid;
+    const uint64_t lanes = (uint64_t)grid * render_block_size();
+    uint64_t total = 0;
+    for (uint64_t w : b->work) total += w;
+    const double want = std::max(1.0, (double)lanes * 16.0 * b->unit_scale);
+    size_t poff = 0;
+    std::vector<size_t> pbase(nj, 0);
+    for (uint32_t j = 0; j < nj; ++j) {
+        const drk_batch_job &J = b->jobs[j];
+        const drk_kernel *k = J.kernel;
+        RenderParams &P = b->hP[j];
+        WorkParams &W = b->hW[j];
+        P = k->P;
+        uint32_t rx, ry;
+        rendered_extent(k, J.x0, J.y0, J.tilex, J.tiley, rx, ry);
+        P.x0 = J.x0;
+        P.y0 = J.y0;
+        P.rx = rx;
+        P.ry = ry;
+        P.tilex = J.tilex;
+        P.samples = J.samples;
+        P.sample_base = J.sample_base;
+        P.key1 = rng_key1(J.seed);
+        P.shadow_mode = 0;
+        P.depth = k->depthcomp ? k->d_depth : nullptr;
+        std::memset(&W, 0, sizeof(W));
+        const uint32_t npix = rx * ry;
+        uint32_t chunk = J.samples;
+        if (npix) {
+            if (k->chunk_override) {
+                chunk = k->chunk_override;
+            } else {
+                // Work units in proportion to the job's share of the samples.
+                const double units = want * (double)b->work[j] / (double)std::max<uint64_t>(total, 1);
+                uint64_t nch = (uint64_t)std::ceil(units / npix);
+                nch = std::max<uint64_t>(1, std::min<uint64_t>(nch, J.samples));
+                chunk = (uint32_t)((J.samples + nch - 1) / nch);
+            }
+            chunk = std::max(1u, std::min(chunk, J.samples));
+        }
+        const uint32_t nchunks = (J.samples + chunk - 1) / chunk;
+        if ((uint64_t)npix * nchunks > 0xffffffffull) return set_error(DRK_EINVAL, "batch: job %u has too many work units", j);
+        W.counter = b->d_counters + (size_t)j * kMaxBands * kCounterStride;
+        W.nbands = k->bands;
+        W.npix = npix;
+        W.chunk = chunk;
+        W.nchunks = nchunks;
+        W.total_units = npix * nchunks;
+        W.out_l = J.d_lights;
+        W.out_e = J.d_env;
+        W.sq_l = b->sumsq ? J.d_sumsq_lights : nullptr;
+        W.sq_e = b->sumsq ? J.d_sumsq_env : nullptr;
+        W.counts = k->d_counts;
+        W.grad_copies = 1;
+        W.grad_nxcc = k->xcc;
+        if (nchunks > 1 && npix) {
+            pbase[j] = poff;
+            poff += (size_t)(b->sumsq ? 4 : 2) * nchunks * npix * 16;
+        }
+    }
+    if (poff > b->partial_bytes) {
+        if (b->d_partial) (void)hipFree(b->d_partial);
+        b->d_partial = nullptr;
+        b->partial_bytes = 0;
+        HIP_TRY(hipMalloc(&b->d_partial, poff), "batch: partial sums");
+        b->partial_bytes = poff;
+    }
+    for (uint32_t j = 0; j < nj; ++j)
+        if (b->hW[j].nchunks > 1 && b->hW[j].npix)
+            b->hW[j].partial = reinterpret_cast<float *>(reinterpret_cast<char *>(b->d_partial) + pbase[j]);
+    HIP_TRY(hipMemcpy(b->dP, b->hP.data(), sizeof(RenderParams) * nj, hipMemcpyHostToDevice), "batch: job table");
+    HIP_TRY(hipMemcpy(b->dW, b->hW.data(), sizeof(WorkParams) * nj, hipMemcpyHostToDevice), "batch: job table");
+    b->planned = true;
+    batch_drop_graph(b);
+    return DRK_OK;
+}
+
+// The launch sequence: clears of the tile pixels outside each rendered
+// extent, the batched render, then the per-job chunk reductions. Kernels
+// only (no memset nodes), so it can be captured into a graph.
+static int batch_enqueue(drk_batch *b, hipStream_t s) {
+    const uint32_t nj = (uint32_t)b->jobs.size();
+    const int mode = b->sumsq ? kModeForwardSq : kModeForward;
+    for (uint32_t j = 0; j < nj; ++j) {
+        const drk_batch_job &J = b->jobs[j];
+        const RenderParams &P = b->hP[j];
+        if (P.rx != J.tilex || P.ry != J.tiley) {
+            const size_t n = (size_t)J.tilex * J.tiley * 4;
+            HIP_TRY(launch_fill(J.d_lights, n, 0.0f, s), "batch: clear");
+            HIP_TRY(launch_fill(J.d_env, n, 0.0f, s), "batch: clear");
+            if (b->sumsq) {
+                HIP_TRY(launch_fill(J.d_sumsq_lights, n, 0.0f, s), "batch: clear");
+                HIP_TRY(launch_fill(J.d_sumsq_env, n, 0.0f, s), "batch: clear");
+            }
+        }
+    }
+    HIP_TRY(launch_render_batch(b->dP, b->dW, nj, b->d_counters, nj * kMaxBands * kCounterStride, b->grid_eff, mode, s),
+            "batch: launch");
+    for (uint32_t j = 0; j < nj; ++j)
+        HIP_TRY(launch_reduce(b->hW[j], b->hP[j].rx, b->hP[j].tilex, b->sumsq ? 4u : 2u, s), "batch: reduce");
+    return DRK_OK;
+}
+
+int drk_batch_create(const drk_batch_job *jobs, uint32_t njobs, drk_batch **out) {
+    if (!out) return set_error(DRK_EINVAL, "drk_batch_create: out is NULL");
+    *out = nullptr;
+    if (!jobs || njobs == 0) return set_error(DRK_EINVAL, "drk_batch_create: no jobs");
+    bool sumsq = jobs[0].d_sumsq_lights != nullptr;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const drk_batch_job &J = jobs[j];
+        if (!J.kernel || !J.d_lights || !J.d_env) return set_error(DRK_EINVAL, "drk_batch_create: job %u: NULL argument", j);
+        if ((J.d_sumsq_lights != nullptr) != sumsq || (J.d_sumsq_env != nullptr) != sumsq)
+            return set_error(DRK_EINVAL, "drk_batch_create: pass sum-of-squares buffers for every job or for none");
+        if (J.tilex == 0 || J.tiley == 0 || J.tilex > J.kernel->tilex || J.tiley > J.kernel->tiley)
+            return set_error(DRK_EINVAL, "drk_batch_create: job %u: tile %ux%u exceeds its kernel's %ux%u", j, J.tilex,
+                             J.tiley, J.kernel->tilex, J.kernel->tiley);
+        if (J.samples == 0) return set_error(DRK_EINVAL, "drk_batch_create: job %u: samples must be > 0", j);
+        if ((uint64_t)J.sample_base + J.samples > 0xffffffffull || (uint64_t)J.x0 + J.tilex > 0xffffffffull ||
+            (uint64_t)J.y0 + J.tiley > 0xffffffffull)
+            return set_error(DRK_EINVAL, "drk_batch_create: job %u: counter or origin overflows", j);
+    }
+    drk_batch *b = new (std::nothrow) drk_batch;
+    if (!b) return set_error(DRK_ENOMEM, "drk_batch_create: host allocation");
+    b->jobs.assign(jobs, jobs + njobs);
+    b->hP.resize(njobs);
+    b->hW.resize(njobs);
+    b->sumsq = sumsq;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        uint32_t rx, ry;
+        rendered_extent(jobs[j].kernel, jobs[j].x0, jobs[j].y0, jobs[j].tilex, jobs[j].tiley, rx, ry);
+        b->work.push_back((uint64_t)rx * ry * jobs[j].samples);
+    }
+    b->cu_count = jobs[0].kernel->cu_count;
+    hipError_t e = hipMalloc(&b->dP, sizeof(RenderParams) * njobs);
+    if (e == hipSuccess) e = hipMalloc(&b->dW, sizeof(WorkParams) * njobs);
+    if (e == hipSuccess) e = hipMalloc(&b->d_counters, sizeof(uint32_t) * njobs * kMaxBands * kCounterStride);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        drk_batch_destroy(b);
+        return hip_fail(e, "drk_batch_create");
+    }
+    int rc = batch_plan(b);
+    if (rc != DRK_OK) {
+        drk_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return DRK_OK;
+}
+
+int drk_batch_set_tuning(drk_batch *b, uint32_t grid_blocks, float unit_scale) {
+    if (!b) return set_error(DRK_EINVAL, "drk_batch_set_tuning: NULL batch");
+    if (!(unit_scale > 0.0f)) return set_error(DRK_EINVAL, "drk_batch_set_tuning: unit_scale must be > 0");
+    b->grid = grid_blocks;
+    b->unit_scale = unit_scale;
+    return batch_plan(b);
+}
+
+int drk_batch_launch(drk_batch *b, void *stream) {
+    if (!b) return set_error(DRK_EINVAL, "drk_batch_launch: NULL batch");
+    return batch_enqueue(b, (hipStream_t)stream);
+}
+
+int drk_batch_launch_graph(drk_batch *b, void *stream) {
+    if (!b) return set_error(DRK_EINVAL, "drk_batch_launch_graph: NULL batch");
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "batch: capture");
+        int rc = batch_enqueue(b, b->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+        if (rc != DRK_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIP_TRY(e, "batch: end capture");
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "batch: instantiate");
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, s), "batch: graph launch");
+    return DRK_OK;
+}
+
+int drk_batch_autotune(drk_batch *b, void *stream, uint32_t reps, char *report, uint32_t report_len) {
+    if (!b) return set_error(DRK_EINVAL, "drk_batch_autotune: NULL batch");
+    hipStream_t s = (hipStream_t)stream;
+    reps = std::max(1u, reps);
+    const int mode = b->sumsq ? kModeForwardSq : kModeForward;
+    const uint32_t base_grid = b->cu_count * (uint32_t)std::max(1, render_batch_blocks_per_cu(mode));
+    const uint32_t grids[2] = {base_grid, base_grid / 2 > 0 ? base_grid / 2 : 1};
+    const float scales[5] = {0.25f, 0.5f, 1.0f, 2.0f, 4.0f};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "autotune: event");
+    HIP_TRY(hipEventCreate(&e1), "autotune: event");
+    std::string log = "grid unit_scale ms\n";
+    float best = 1e30f;
+    uint32_t best_grid = base_grid;
+    float best_scale = 1.0f;
+    int rc = DRK_OK;
+    for (uint32_t g : grids) {
+        for (float sc : scales) {
+            b->grid = g;
+            b->unit_scale = sc;
+            if ((rc = batch_plan(b)) != DRK_OK) break;
+            if ((rc = batch_enqueue(b, s)) != DRK_OK) break;   // warm-up
+            (void)hipEventRecord(e0, s);
+            for (uint32_t r = 0; r < reps && rc == DRK_OK; ++r) rc = batch_enqueue(b, s);
+            (void)hipEventRecord(e1, s);
+            if (rc != DRK_OK) break;
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= reps;
+            char line[96];
+            std::snprintf(line, sizeof(line), "%u %.2f %.3f\n", g, sc, ms);
+            log += line;
+            if (ms < best) {
+                best = ms;
+                best_grid = g;
+                best_scale = sc;
+            }
+        }
+        if (rc != DRK_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != DRK_OK) return rc;
+    b->grid = best_grid;
+    b->unit_scale = best_scale;
+    char line[96];
+    std::snprintf(line, sizeof(line), "chosen %u %.2f %.3f\n", best_grid, best_scale, best);
+    log += line;
+    if (report && report_len) std::snprintf(report, report_len, "%s", log.c_str());
+    return batch_plan(b);
+}
+
+void drk_batch_destroy(drk_batch *b) {
+    if (!b) return;
+    if (b->cap_stream) (void)hipStreamSynchronize(b->cap_stream);
+    (void)hipDeviceSynchronize();
+    batch_drop_graph(b);
+    if (b->dP) (void)hipFree(b->dP);
+    if (b->dW) (void)hipFree(b->dW);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    if (b->d_partial) (void)hipFree(b->d_partial);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
 }
 
 }  // extern "C"

@@ -71,23 +71,34 @@ __device__ __forceinline__ uint32_t fetch_units(uint32_t *counter, bool need) {
     return base + below;
 }
 
+// Workgroup-shared state of the path tracer.
+struct RenderLds {
+    MieLds mie;
+    uint32_t stk[kMaxDepth * kBlock];   // per-lane traversal stacks, lane-interleaved
+    // Adjoint: per wave, the next free slot and the end of the chunk of the
+    // event buffer it has reserved (in LDS, so that every lane sees the
+    // updates made under divergent control flow).
+    unsigned long long ev_cur[kBlock / 64][2];
+};
+
+__device__ __forceinline__ void render_lds_init(RenderLds &S) {
+    if ((threadIdx.x & 63) == 0) S.ev_cur[threadIdx.x >> 6][0] = S.ev_cur[threadIdx.x >> 6][1] = 0;
+    load_mie(S.mie);
+    __syncthreads();
+}
+
+// Runs one launch's work (P, W) to completion in the calling wave: its lanes
+// take work units until the units are drained, then the wave returns.
 template <int kMode>
-__device__ __forceinline__ void render_body(const RenderParams &P, const WorkParams &W) {
+__device__ __forceinline__ void render_body(const RenderParams &P, const WorkParams &W, RenderLds &S) {
     constexpr bool kSumSq = (kMode == kModeForwardSq || kMode == kModeCount);
     constexpr bool kCount = (kMode == kModeCount);
     constexpr bool kAdj = (kMode == kModeAdjoint);
     constexpr bool kRec = (kMode == kModeRecord);
-    __shared__ MieLds mie;
-    __shared__ uint32_t stk_all[kMaxDepth * kBlock];
-    // Adjoint: per wave, the next free slot and the end of the chunk of the
-    // event buffer it has reserved (in LDS, so that every lane sees the
-    // updates made under divergent control flow).
-    __shared__ unsigned long long ev_cur[kBlock / 64][2];
-    if (kAdj && (threadIdx.x & 63) == 0) ev_cur[threadIdx.x >> 6][0] = ev_cur[threadIdx.x >> 6][1] = 0;
-    load_mie(mie);
-    __syncthreads();
+    MieLds &mie = S.mie;
+    unsigned long long (*ev_cur)[2] = S.ev_cur;
 
-    uint32_t *stk = stk_all + threadIdx.x;
+    uint32_t *stk = S.stk + threadIdx.x;
     const uint32_t stride = kBlock;
     const KernelConsts &K = *P.kc;
     const float sig_s = P.sigma_s;
@@ -568,14 +579,35 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
 template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W) {
-    render_body<kMode>(P, W);
+    __shared__ RenderLds S;
+    render_lds_init(S);
+    render_body<kMode>(P, W, S);
 }
 
 // The adjoint carries more live state: held to 168 VGPRs so that three
 // waves per SIMD stay resident, as in the forward instantiations.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render_adjoint(RenderParams P,
                                                                                                     WorkParams W) {
-    render_body<kModeAdjoint>(P, W);
+    __shared__ RenderLds S;
+    render_lds_init(S);
+    render_body<kModeAdjoint>(P, W, S);
+}
+
+// Many launches (scenes, views, tiles) in one persistent grid. Workgroup b
+// starts at job b % njobs and visits every job once; each of its waves drains
+// the job's work units (shared with every other workgroup on that job) before
+// moving on, so the job index is wave-uniform and the job's parameters are
+// read with scalar loads.
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_render_batch(const RenderParams *__restrict__ Ps,
+                                                         const WorkParams *__restrict__ Ws, uint32_t njobs) {
+    __shared__ RenderLds S;
+    render_lds_init(S);
+    const uint32_t j0 = blockIdx.x % njobs;
+    for (uint32_t i = 0; i < njobs; ++i) {
+        const uint32_t j = (j0 + i < njobs) ? j0 + i : j0 + i - njobs;
+        render_body<kMode>(Ps[j], Ws[j], S);
+    }
 }
 
 // Sums the per-chunk partials of each pixel in chunk order (deterministic).
@@ -624,6 +656,33 @@ int render_blocks_per_cu(int mode) {
     }
     if (e != hipSuccess || nb <= 0) return 1;
     return nb;
+}
+
+int render_batch_blocks_per_cu(int mode) {
+    int nb = 0;
+    hipError_t e = mode == kModeForwardSq
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeForwardSq>, kBlock, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeForward>, kBlock, 0);
+    if (e != hipSuccess || nb <= 0) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb;
+}
+
+hipError_t launch_render_batch(const RenderParams *dP, const WorkParams *dW, uint32_t njobs, uint32_t *counters,
+                               uint32_t ncounters, uint32_t grid, int mode, hipStream_t s) {
+    if (njobs == 0) return hipSuccess;
+    k_fill_u32<<<(ncounters + 255) / 256, 256, 0, s>>>(counters, ncounters, 0u);
+    if (mode == kModeForwardSq) k_render_batch<kModeForwardSq><<<grid, kBlock, 0, s>>>(dP, dW, njobs);
+    else k_render_batch<kModeForward><<<grid, kBlock, 0, s>>>(dP, dW, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const WorkParams &W, uint32_t rx, uint32_t tilex, uint32_t buffers, hipStream_t s) {
+    if (!W.partial || W.npix == 0) return hipSuccess;
+    k_reduce<<<(W.npix + 255) / 256, 256, 0, s>>>(W, rx, tilex, buffers);
+    return hipGetLastError();
 }
 
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,

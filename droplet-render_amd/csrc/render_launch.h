@@ -59,6 +59,13 @@ int render_blocks_per_cu(int mode);
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+// Batched forward launch over njobs (P, W) pairs in device memory; counters
+// (ncounters words, every job's band counters) are cleared first.
+int render_batch_blocks_per_cu(int mode);
+hipError_t launch_render_batch(const RenderParams *dP, const WorkParams *dW, uint32_t njobs, uint32_t *counters,
+                               uint32_t ncounters, uint32_t grid, int mode, hipStream_t s);
+// Chunk-order sum of a launch's partials into its outputs (no-op without partials).
+hipError_t launch_reduce(const WorkParams &W, uint32_t rx, uint32_t tilex, uint32_t buffers, hipStream_t s);
 // out[i] += sum over c < copies of part[c * n + i].
 hipError_t launch_grad_sum(const float *part, uint32_t copies, size_t n, float *out, hipStream_t s);
 

@@ -227,6 +227,39 @@ int drk_kernel_adjoint_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, ui
                                      const void *d_record, size_t record_bytes, void *stream);
 
 /*
+ * Batched forward launch (no reference counterpart): many render calls --
+ * scenes, views, tiles of uneven size, each a (kernel, tile) job writing its
+ * own device sum buffers as drk_kernel_render_device would -- run in one
+ * persistent grid. Each workgroup visits every job once, starting at a
+ * different one; its waves drain a job's work units before moving on.
+ * drk_batch_create uploads the job table (jobs may share kernels and
+ * scenes; buffers must not overlap). drk_batch_launch enqueues the batch on
+ * a hipStream_t (NULL: the default stream). drk_batch_launch_graph captures
+ * the same launch sequence into a hipGraph on its first call and replays
+ * the graph after that (a graph made before drk_batch_set_tuning or
+ * drk_batch_autotune is rebuilt). drk_batch_set_tuning sets the grid
+ * (workgroups; 0 = occupancy) and the work units per lane relative to the
+ * default; drk_batch_autotune times a grid x units sweep on the stream
+ * (synchronising it), keeps the fastest and writes the table to report.
+ * Per-job results equal those of separate launches with the same chunking.
+ */
+typedef struct drk_batch drk_batch;
+typedef struct drk_batch_job {
+    drk_kernel *kernel;
+    uint32_t x0, y0, tilex, tiley, samples;
+    uint64_t seed;
+    uint32_t sample_base;
+    float *d_lights, *d_env;               /* tilex*tiley float4 sums each */
+    float *d_sumsq_lights, *d_sumsq_env;   /* NULL for every job, or set for every job */
+} drk_batch_job;
+int drk_batch_create(const drk_batch_job *jobs, uint32_t njobs, drk_batch **out);
+int drk_batch_set_tuning(drk_batch *b, uint32_t grid_blocks, float unit_scale);
+int drk_batch_launch(drk_batch *b, void *stream);
+int drk_batch_launch_graph(drk_batch *b, void *stream);
+int drk_batch_autotune(drk_batch *b, void *stream, uint32_t reps, char *report, uint32_t report_len);
+void drk_batch_destroy(drk_batch *b);
+
+/*
  * Extensions with no reference counterpart.
  *
  * drk_kernel_set_tuning: fixes the work-unit chunk (samples per unit) and the

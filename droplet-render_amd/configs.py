@@ -130,3 +130,41 @@ def c2(nscenes=32, seed=2, max_n=512):
         g = _cloud_with_occupancy(n, 1000 + i, occ, i % 2)
         out.append(Workload("c2_%02d_n%d" % (i, n), g, Scene.from_grid(g), res, res, spp, 16, 16.0))
     return out
+
+
+@dataclass
+class FitWorkload:
+    """Differentiable density fitting: a ground-truth cloud, the scene being
+    fitted (its own octree and bricks) and the ring of views."""
+    gt: Workload
+    init: Scene
+    views: list
+    w: int
+    h: int
+    spp: int
+    scattevs: int
+    sigma_t: float
+
+
+def ring_views(n, elevation_deg=20.0, radius=2.6, center=(0.5, 0.5, 0.5), w=256, h=256, fov=40.0):
+    """`n` cameras on a ring at `elevation_deg` around `center`, looking at it."""
+    out = []
+    el = np.radians(elevation_deg)
+    for i in range(n):
+        az = 2.0 * np.pi * i / n
+        eye = (center[0] + radius * np.cos(az) * np.cos(el), center[1] + radius * np.sin(az) * np.cos(el),
+               center[2] + radius * np.sin(el))
+        out.append(camera(eye, center, fov, w, h))
+    return out
+
+
+def c3(n=128, views=16, res=256, spp=32, scattevs=8, sigma_t=8.0, seed=7, init_density=0.5):
+    """Density fitting: ground truth a 128^3 fBm cloud (seed 7); the fitted
+    scene starts as a uniform density 0.5 (of sigma_t) over the whole grid,
+    with majorants 1 so that densities may grow to 1; 16 views on a ring at
+    20 degrees elevation, 256^2, 32 spp, 8 scattering events."""
+    g = synth_fbm(n, seed)
+    gt = Workload("c3_gt_fbm%d" % n, g, Scene.from_grid(g), res, res, spp, scattevs, sigma_t)
+    init = Scene.from_grid(np.full((n, n, n), init_density, np.float32))
+    init.raise_majorants(1.0, floor=1.0)
+    return FitWorkload(gt, init, ring_views(views, w=res, h=res), res, res, spp, scattevs, sigma_t)

@@ -66,6 +66,12 @@ def main():
     if args.workload == "c2":
         bench_batch(pkg, torch, args)
         return
+    if args.workload == "c3":
+        bench_fit(pkg, torch, args)
+        return
+    if args.workload == "c4":
+        bench_large(pkg, torch, args)
+        return
 
     t_build = time.perf_counter()
     wl = getattr(pkg.configs, args.workload)()
@@ -289,6 +295,119 @@ def bench_batch(pkg, torch, args):
     }
     b.destroy()
     for k in kernels:
+        k.Destroy()
+    print(json.dumps(line), flush=True)
+
+
+def bench_fit(pkg, torch, args):
+    """Differentiable density fitting (128^3 fBm ground truth, 16 ring views,
+    256^2, 32 spp, 8 scattering events): Adam iterations, each a forward with
+    path record and an adjoint per view; also the forward alone."""
+    from droplet_render_amd.fit import DensityFit
+    t_build = time.perf_counter()
+    fw = pkg.configs.c3()
+    fit = DensityFit(fw)
+    t_build = time.perf_counter() - t_build
+    samples = len(fw.views) * fw.w * fw.h * fw.spp
+    rho = torch.tensor(fw.init.pvol[1], device="cuda")
+    rho, losses = fit.run(args.warmup, seed=1000, rho=rho)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho, more = fit.run(args.steps, seed=2000, rho=rho)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    losses += more
+    with torch.no_grad():
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(args.steps):
+            fit.loss(rho, seed=3000 + i)
+        torch.cuda.synchronize()
+        fwd = (time.perf_counter() - t1) / args.steps
+    it = wall / args.steps
+    line = {
+        "metric": METRIC + " (density-fitting variant)",
+        "value": round(samples / it / 1e6, 3), "unit": "Msamples/s (forward+adjoint)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(it * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c3: fit a uniform 0.5 start to a 128^3 fBm cloud (seed 7) from 16 views on a "
+                               "20-degree ring, 256^2, 32 spp, 8 scattering events, L2 image loss, Adam lr 0.05",
+                   "samples_per_step": samples, "setup_s": round(t_build, 2)},
+        "forward_only_ms_per_step": round(fwd * 1e3, 3),
+        "ratio_to_forward": round(it / fwd, 3),
+        "loss_first_last": [round(losses[0], 4), round(losses[-1], 4)],
+    }
+    fit.destroy()
+    print(json.dumps(line), flush=True)
+
+
+def bench_large(pkg, torch, args):
+    """Large forward+adjoint: 512^3 sparse cloud, 8 views at 1920x1080, 16 spp,
+    64 scattering events. Forward: the 8 views in one batched launch. Adjoint:
+    per view, from the path record, into one FP32 gradient. Reports the device
+    footprint and the algorithmic GB/s of the forward."""
+    t_build = time.perf_counter()
+    wl, views = pkg.configs.c4()
+    t_build = time.perf_counter() - t_build
+    from droplet_render_amd.fit import _kernel
+    ks = [_kernel(wl.scene, vi, pi, wl.w, wl.h, wl.scattevs, wl.sigma_t) for vi, pi in views]
+    n = wl.w * wl.h * 4
+    outs = [(torch.empty(n, device="cuda"), torch.empty(n, device="cuda")) for _ in ks]
+    jobs = [(k, 0, 0, wl.w, wl.h, wl.spp, args.seed + i, 0, o[0], o[1]) for i, (k, o) in enumerate(zip(ks, outs))]
+    b = pkg.batch.Batch(jobs)
+    samples = len(ks) * wl.samples
+    b.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.launch()
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / args.steps
+    # the same forward as separate launches (the fwd+adjoint leg is per view)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for i, k in enumerate(ks):
+        k.render_device(0, 0, wl.w, wl.h, wl.spp, args.seed + i, 0, outs[i][0], outs[i][1])
+    torch.cuda.synchronize()
+    fwd_sep = time.perf_counter() - t2
+    # forward with path records + adjoint per view (dense random image gradient)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    adj_l = (torch.rand(n, generator=g) * 1000.0).cuda()
+    adj_e = torch.rand(n, generator=g).cuda()
+    grad = torch.zeros(wl.scene.leafx[1] * wl.scene.lvoxc3, device="cuda")
+    rb = ks[0].record_bytes(0, 0, wl.w, wl.h, wl.spp)
+    recs = [torch.empty(rb // 4, device="cuda") for _ in ks]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i, k in enumerate(ks):
+        k.render_record_device(0, 0, wl.w, wl.h, wl.spp, args.seed + i, 0, outs[i][0], outs[i][1], recs[i])
+    for i, k in enumerate(ks):
+        k.adjoint_record_device(0, 0, wl.w, wl.h, wl.spp, args.seed + i, 0, adj_l, adj_e, grad, recs[i])
+    torch.cuda.synchronize()
+    fa = time.perf_counter() - t1
+    # algorithmic bytes of one view's forward (counting pass)
+    nn, nl = ks[0].count(0, 0, wl.w, wl.h, wl.spp, args.seed, 0)
+    alg = 64.0 * nn + 32.0 * nl + 32.0 * wl.w * wl.h
+    footprint = wl.scene.footprint() + sum(k.footprint() for k in ks) + rb * len(recs) + 4 * grad.numel() + \
+        2 * 4 * n * len(outs)
+    line = {
+        "metric": METRIC + " (large fwd+adjoint variant)",
+        "value": round(samples / fwd / 1e6, 3), "unit": "Msamples/s (forward)", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(fwd * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c4: 512^3 sparse cloud (3 fBm-displaced spheres, %.1f%% of the leaf lattice "
+                               "occupied), sigma_t 32, 8 views 1920x1080 on a ring, 16 spp, 64 scattering events"
+                               % (100 * wl.scene.occupancy()),
+                   "samples_per_step": samples, "bricks": wl.scene.leafx[1], "scene_build_s": round(t_build, 1)},
+        "fwd_adjoint": {"value": round(samples / fa / 1e6, 3), "unit": "Msamples/s", "ms_per_step": round(fa * 1e3, 3),
+                        "ratio_to_forward": round(fa / fwd_sep, 3),
+                        "forward_separate_launches_ms": round(fwd_sep * 1e3, 3),
+                        "gradient_finite": bool(torch.isfinite(grad).all().item())},
+        "forward_algorithmic_GBps": round(alg * len(ks) / fwd / 1e9, 1),
+        "device_footprint_bytes": int(footprint),
+    }
+    b.destroy()
+    for k in ks:
         k.Destroy()
     print(json.dumps(line), flush=True)
 

@@ -168,3 +168,15 @@ def c3(n=128, views=16, res=256, spp=32, scattevs=8, sigma_t=8.0, seed=7, init_d
     init = Scene.from_grid(np.full((n, n, n), init_density, np.float32))
     init.raise_majorants(1.0, floor=1.0)
     return FitWorkload(gt, init, ring_views(views, w=res, h=res), res, res, spp, scattevs, sigma_t)
+
+
+def c4(n=512, views=8, w=1920, h=1080, spp=16, scattevs=64, sigma_t=32.0, seed=3):
+    """Large forward+adjoint: a 512^3 sparse cloud (3 seeded spheres displaced
+    by 6-octave fBm; about 2% of the leaf lattice holds density), sigma_t 32,
+    8 views at 1920x1080 on a 20-degree ring, 16 spp, 64 scattering events.
+    Returns (workload of view 0, list of (viewi, proji) for all views)."""
+    g = synth_spheres(n, seed, spheres=3, rmin=0.03, rmax=0.06, amp=0.05, octaves=6)
+    vs = ring_views(views, w=w, h=h, radius=1.6, fov=30.0)
+    wl = Workload("c4_spheres%d" % n, g, Scene.from_grid(g), w, h, spp, scattevs, sigma_t,
+                  viewi=vs[0][0], proji=vs[0][1])
+    return wl, vs

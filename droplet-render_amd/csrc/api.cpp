@@ -53,10 +53,18 @@ struct drk_scene {
     uint32_t leafx[2] = {0, 0};
     uint32_t lvoxc = 0;
     int device = 0;
+    // Adjoint gradient events, shared by the scene's kernels (adjoint calls
+    // on kernels of one scene run one at a time): budget (bytes; 0: direct
+    // atomics only), slots allocated, and the buffers (events + sorted copy,
+    // bin tables).
+    size_t ev_budget = SIZE_MAX;    // SIZE_MAX: a quarter of device memory
+    uint32_t ev_cap = 0;
+    void *d_ev = nullptr;
+    uint32_t *d_evbins = nullptr;   // count (2 words) + hist + bin_start + bin_fill + blk_start
 };
 
 struct drk_kernel {
-    const drk_scene *scene = nullptr;
+    drk_scene *scene = nullptr;     // shares its event buffers with the scene's other kernels
     RenderParams P;                 // launch template (tile fields filled per call)
     hipStream_t stream = nullptr;
     uint32_t tilex = 0, tiley = 0;
@@ -76,12 +84,6 @@ struct drk_kernel {
     uint32_t grad_copies = 1;       // gradient copies of the adjoint (one per XCD by default)
     float *d_gradx = nullptr;       // grad_copies * (FOG gradient size) floats
     size_t gradx_bytes = 0;
-    // Adjoint gradient events: budget (bytes; 0: direct atomics only), slots
-    // allocated, and the buffers (events, sorted copy, bin tables).
-    size_t ev_budget = SIZE_MAX;    // SIZE_MAX: a quarter of device memory
-    uint32_t ev_cap = 0;
-    void *d_ev = nullptr;
-    uint32_t *d_evbins = nullptr;   // count (2 words) + hist + bin_start + bin_fill + blk_start
     float *d_depth = nullptr;
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
@@ -231,6 +233,8 @@ void drk_scene_destroy(drk_scene *s) {
     if (s->d_chx) (void)hipFree(s->d_chx);
     if (s->d_fog) (void)hipFree(s->d_fog);
     if (s->d_sdf) (void)hipFree(s->d_sdf);
+    if (s->d_ev) (void)hipFree(s->d_ev);
+    if (s->d_evbins) (void)hipFree(s->d_evbins);
     delete s;
 }
 
@@ -276,7 +280,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     if (!k) return set_error(DRK_ENOMEM, "drk_kernel_create: host allocation");
     std::memset(&k->P, 0, sizeof(k->P));
     RenderParams &P = k->P;
-    k->scene = scene;
+    k->scene = const_cast<drk_scene *>(scene);
     k->tilex = tilex;
     k->tiley = tiley;
     P.ob = scene->d_ob;
@@ -496,30 +500,31 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         // calls and reallocated when a call wants more.
         uint32_t shift = 0, nbins = 0;
         const bool binnable = event_bins(k->scene->leafx[1], k->P.lvoxc3, shift, nbins);
-        size_t budget = k->ev_budget;
+        drk_scene *sc = k->scene;
+        size_t budget = sc->ev_budget;
         if (budget == SIZE_MAX) {
             size_t fr = 0, tot = 0;
             budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : 0;
         }
         const uint64_t want = std::min<uint64_t>((uint64_t)npix * samples * 16, 0xffffffffull);
         const uint64_t cap = binnable ? (std::min<uint64_t>(want, budget / kEventSlotBytes) & ~63ull) : 0;
-        if (cap > k->ev_cap) {
-            if (k->d_ev) (void)hipFree(k->d_ev);
-            if (k->d_evbins) (void)hipFree(k->d_evbins);
-            k->d_ev = nullptr;
-            k->d_evbins = nullptr;
-            k->ev_cap = 0;
-            HIP_TRY(hipMalloc(&k->d_ev, (size_t)cap * kEventSlotBytes), "adjoint: event buffer");
-            HIP_TRY(hipMalloc(&k->d_evbins, (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t)), "adjoint: event bins");
-            k->ev_cap = (uint32_t)cap;
+        if (cap > sc->ev_cap) {
+            if (sc->d_ev) (void)hipFree(sc->d_ev);
+            if (sc->d_evbins) (void)hipFree(sc->d_evbins);
+            sc->d_ev = nullptr;
+            sc->d_evbins = nullptr;
+            sc->ev_cap = 0;
+            HIP_TRY(hipMalloc(&sc->d_ev, (size_t)cap * kEventSlotBytes), "adjoint: event buffer");
+            HIP_TRY(hipMalloc(&sc->d_evbins, (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t)), "adjoint: event bins");
+            sc->ev_cap = (uint32_t)cap;
         }
         if (cap > 0) {
-            const uint32_t c = k->ev_cap;
-            W.ev_base = static_cast<char *>(k->d_ev);
-            W.ev_count = reinterpret_cast<unsigned long long *>(k->d_evbins);
+            const uint32_t c = sc->ev_cap;
+            W.ev_base = static_cast<char *>(sc->d_ev);
+            W.ev_count = reinterpret_cast<unsigned long long *>(sc->d_evbins);
             W.ev_cap = c;
             E.ev_base = W.ev_base;
-            E.bins = k->d_evbins;
+            E.bins = sc->d_evbins;
             E.ev_cap = c;
             E.nbins = nbins;
             E.shift = shift;
@@ -527,7 +532,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             E.lvoxc = k->P.lvoxc;
             E.lvoxc3 = k->P.lvoxc3;
             E.grad = adj->grad;
-            HIP_TRY(launch_fill(reinterpret_cast<float *>(k->d_evbins), 2, 0.0f, s), "adjoint: clear");
+            HIP_TRY(launch_fill(reinterpret_cast<float *>(sc->d_evbins), 2, 0.0f, s), "adjoint: clear");
         }
     }
     if (adj) {
@@ -736,7 +741,6 @@ int drk_kernel_footprint(const drk_kernel *k, size_t *bytes) {
     if (!k || !bytes) return set_error(DRK_EINVAL, "drk_kernel_footprint: NULL argument");
     size_t b = (size_t)4 * k->tilex * k->tiley * 16 + k->partial_bytes + k->gradx_bytes + sizeof(KernelConsts);
     b += kMaxBands * kCounterStride * sizeof(uint32_t) + 64;
-    if (k->d_ev) b += (size_t)k->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
     if (k->d_env) b += (size_t)k->C.env_w * k->C.env_h * 16;
     if (k->d_depth) b += (size_t)k->P.w * k->P.h * sizeof(float);
     *bytes = b;
@@ -747,17 +751,20 @@ int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
     if (!s || !bytes) return set_error(DRK_EINVAL, "drk_scene_footprint: NULL argument");
     *bytes = (size_t)s->node_count * (sizeof(drk_octree_node) + 8 * sizeof(uint32_t)) +
              (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
+    if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
     return DRK_OK;
 }
 
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes) {
     if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_event_budget: NULL kernel");
-    k->ev_budget = bytes == UINT64_MAX ? SIZE_MAX : (size_t)bytes;
-    if (k->d_ev) (void)hipFree(k->d_ev);
-    if (k->d_evbins) (void)hipFree(k->d_evbins);
-    k->d_ev = nullptr;
-    k->d_evbins = nullptr;
-    k->ev_cap = 0;
+    drk_scene *sc = k->scene;
+    sc->ev_budget = bytes == UINT64_MAX ? SIZE_MAX : (size_t)bytes;
+    if (sc->d_ev || sc->d_evbins) (void)hipDeviceSynchronize();   // a launch may still use them
+    if (sc->d_ev) (void)hipFree(sc->d_ev);
+    if (sc->d_evbins) (void)hipFree(sc->d_evbins);
+    sc->d_ev = nullptr;
+    sc->d_evbins = nullptr;
+    sc->ev_cap = 0;
     return DRK_OK;
 }
 
@@ -801,8 +808,6 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_env) (void)hipFree(k->d_env);
     if (k->d_kc) (void)hipFree(k->d_kc);
     if (k->d_gradx) (void)hipFree(k->d_gradx);
-    if (k->d_ev) (void)hipFree(k->d_ev);
-    if (k->d_evbins) (void)hipFree(k->d_evbins);
     if (k->d_depth) (void)hipFree(k->d_depth);
     for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
     for (auto &pr : k->ev_free) {

@@ -202,17 +202,21 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
  * Gradient events (no reference counterpart). The adjoint writes each
  * gradient contribution as a 24-byte event, sorts the events by brick and
  * sums them in on-chip memory, which avoids per-corner float atomics in HBM.
- * The event buffer (48 bytes per slot, 16 slots per pixel sample of a call)
- * is limited to `bytes` (UINT64_MAX: a quarter of device memory, the
- * default; 0: no events, every contribution is added with atomics); events
- * beyond it are added with atomics. The gradient is the same either way up
- * to the order of float additions.
+ * The event buffer belongs to the kernel's scene and is shared by the
+ * scene's kernels, so adjoint calls on kernels of one scene must not overlap
+ * in time. It holds 48 bytes per slot, grows to 16 slots per pixel sample of
+ * the largest call so far, and is limited to `bytes` (UINT64_MAX: a quarter
+ * of device memory, the default; 0: no events, every contribution is added
+ * with atomics); events beyond it are added with atomics. The gradient is the
+ * same either way up to the order of float additions. Setting the budget
+ * frees the current buffer (after a device synchronisation).
  */
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes);
 
-/* Device memory held by a kernel (result, partial-sum, gradient-copy and
- * event buffers as currently allocated) and by a scene (node records, child
- * links, FOG bricks), in bytes. No reference counterpart. */
+/* Device memory held by a kernel (result, partial-sum and gradient-copy
+ * buffers as currently allocated) and by a scene (node records, child links,
+ * FOG bricks, the adjoint's event buffers), in bytes. No reference
+ * counterpart. */
 int drk_kernel_footprint(const drk_kernel *k, size_t *bytes);
 int drk_scene_footprint(const drk_scene *s, size_t *bytes);
 

@@ -79,6 +79,8 @@ _SIGS = {
     "drk_device_info": (C.c_int, [C.c_char_p, u32, _U]),
     "drk_scene_build_dense": (C.c_int, [_F, _F, u32, u32, u32, f32, u32, C.POINTER(C.POINTER(HostScene))]),
     "drk_host_scene_free": (None, [C.POINTER(HostScene)]),
+    "drk_scene_save": (C.c_int, [C.POINTER(HostScene), C.c_char_p]),
+    "drk_scene_load": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(HostScene))]),
     "drk_scene_create": (C.c_int, [C.POINTER(OctreeNode), u32, _F, u32, _F, u32, u32, C.POINTER(_P)]),
     "drk_scene_update_fog": (C.c_int, [_P, _F, _F]),
     "drk_scene_destroy": (None, [_P]),

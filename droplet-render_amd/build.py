@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdroplet_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["api.cpp", "scene_build.cpp", "hosek.cpp", "synth.cpp", "render.hip", "probe.hip"]
+SOURCES = ["api.cpp", "scene_build.cpp", "scene_blob.cpp", "hosek.cpp", "synth.cpp", "render.hip", "probe.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -36,6 +36,11 @@ class Scene:
         s = float(voxel_size) if voxel_size else 1.0 / nx
         out = C.POINTER(L.HostScene)()
         L.check(L.lib().drk_scene_build_dense(L.fptr(fog), None, nx, ny, nz, s, maxd, C.byref(out)))
+        return cls._adopt(out, s)
+
+    @classmethod
+    def _adopt(cls, out, voxel_size=None):
+        """Copies a drk_host_scene into a Scene and frees it."""
         try:
             hs = out.contents
             n = hs.node_count
@@ -44,10 +49,33 @@ class Scene:
             l3 = hs.lvoxc ** 3
             nf = hs.leafx[1] * l3
             pf = (np.ctypeslib.as_array(hs.pvol[1], shape=(nf,)).copy() if nf else np.zeros(0, np.float32))
-            sc = cls(ob, pf, hs.lvoxc, hs.mlevel, np.array(hs.root_ce[:], np.float32), s)
+            sc = cls(ob, pf, hs.lvoxc, hs.mlevel, np.array(hs.root_ce[:], np.float32), voxel_size)
         finally:
             L.lib().drk_host_scene_free(out)
         return sc
+
+    def save(self, path):
+        """Writes the node records and bricks to a scene blob (drk_scene_save)."""
+        hs = L.HostScene()
+        hs.ob = self.ob.ctypes.data_as(C.POINTER(L.OctreeNode))
+        hs.node_count = self.node_count
+        hs.pvol[0] = None
+        hs.pvol[1] = L.fptr(self.pvol[1]) if self.leafx[1] else None
+        hs.leafx[0] = 0
+        hs.leafx[1] = self.leafx[1]
+        hs.lvoxc = self.lvoxc
+        hs.mlevel = self.mlevel or 0
+        rc = self.root_ce if self.root_ce is not None else np.zeros(4, np.float32)
+        for i in range(4):
+            hs.root_ce[i] = float(rc[i])
+        L.check(L.lib().drk_scene_save(C.byref(hs), str(path).encode()))
+
+    @classmethod
+    def load(cls, path):
+        """Reads a scene blob written by `save` (drk_scene_load)."""
+        out = C.POINTER(L.HostScene)()
+        L.check(L.lib().drk_scene_load(str(path).encode(), C.byref(out)))
+        return cls._adopt(out)
 
     @property
     def node_count(self):

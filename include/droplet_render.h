@@ -67,6 +67,17 @@ int drk_scene_build_dense(const float *fog, const float *sdf,
                           drk_host_scene **out);
 void drk_host_scene_free(drk_host_scene *scene);
 
+/*
+ * On-disk scene blob (no reference counterpart; the reference caches the
+ * OpenVDB grids, src/scene.cpp:420-477, and rebuilds the octree each time):
+ * the node records and both brick buffers of a built scene, with a header
+ * and an FNV-1a 64 checksum of the payload. drk_scene_load returns a host
+ * scene (free it with drk_host_scene_free); a file with a bad magic, sizes,
+ * checksum or out-of-range indices is rejected with DRK_EINVAL.
+ */
+int drk_scene_save(const drk_host_scene *scene, const char *path);
+int drk_scene_load(const char *path, drk_host_scene **out);
+
 /* Device-resident scene: uploads Scene::ob and Scene::pvol once
  * (the reference reads them in place, src/kernel.cpp:96-445). */
 typedef struct drk_scene drk_scene;

@@ -146,15 +146,15 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
     if (!ob || node_count == 0 || node_count > kMaxNodes || lvoxc < 2 || lvoxc > 256)
         return set_error(DRK_EINVAL, "drk_scene_create: invalid octree (nodes=%u lvoxc=%u)", node_count, lvoxc);
     if (leafx_fog && !pvol_fog) return set_error(DRK_EINVAL, "drk_scene_create: FOG bricks missing");
-    if (leafx_sdf || pvol_sdf) return set_error(DRK_EINVAL, "drk_scene_create: SDF bricks are not supported yet");
+    if (leafx_sdf && !pvol_sdf) return set_error(DRK_EINVAL, "drk_scene_create: SDF bricks missing");
     // Validate the structure the kernels index blindly.
     for (uint32_t i = 0; i < node_count; ++i) {
         const drk_octree_node &n = ob[i];
         for (int c = 0; c < 8; ++c)
             if (n.chn[c] >= node_count || (n.chn[c] != 0 && n.chn[c] <= i))
                 return set_error(DRK_EINVAL, "drk_scene_create: node %u child %d out of range", i, c);
-        if (n.volx[DRK_VOLUME_SDF] != DRK_NO_BRICK)
-            return set_error(DRK_EINVAL, "drk_scene_create: node %u has an SDF brick (unsupported)", i);
+        if (n.volx[DRK_VOLUME_SDF] != DRK_NO_BRICK && n.volx[DRK_VOLUME_SDF] >= leafx_sdf)
+            return set_error(DRK_EINVAL, "drk_scene_create: node %u SDF brick %u >= leafx %u", i, n.volx[DRK_VOLUME_SDF], leafx_sdf);
         if (n.volx[DRK_VOLUME_FOG] != DRK_NO_BRICK && n.volx[DRK_VOLUME_FOG] >= leafx_fog)
             return set_error(DRK_EINVAL, "drk_scene_create: node %u brick %u >= leafx %u", i, n.volx[DRK_VOLUME_FOG], leafx_fog);
         if (!(n.ce[3] > 0.0f)) return set_error(DRK_EINVAL, "drk_scene_create: node %u has non-positive extent", i);
@@ -179,7 +179,7 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
     if (!s) return set_error(DRK_ENOMEM, "drk_scene_create: host allocation");
     s->node_count = node_count;
     s->h_ob.assign(ob, ob + node_count);
-    s->leafx[0] = 0;
+    s->leafx[0] = leafx_sdf;
     s->leafx[1] = leafx_fog;
     s->lvoxc = lvoxc;
     hipError_t e = hipGetDevice(&s->device);
@@ -203,6 +203,9 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
     size_t nb = (size_t)leafx_fog * lvoxc * lvoxc * lvoxc;
     if (e == hipSuccess && nb) e = hipMalloc(&s->d_fog, nb * sizeof(float));
     if (e == hipSuccess && nb) e = hipMemcpy(s->d_fog, pvol_fog, nb * sizeof(float), hipMemcpyHostToDevice);
+    const size_t ns = (size_t)leafx_sdf * lvoxc * lvoxc * lvoxc;
+    if (e == hipSuccess && ns) e = hipMalloc(&s->d_sdf, ns * sizeof(float));
+    if (e == hipSuccess && ns) e = hipMemcpy(s->d_sdf, pvol_sdf, ns * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         drk_scene_destroy(s);
         return hip_fail(e, "drk_scene_create");
@@ -667,6 +670,8 @@ static int adjoint_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex,
     if (k->P.scattevs > 64) return set_error(DRK_EINVAL, "%s: scattevs > 64 is not supported", fn);
     if (k->depthcomp) return set_error(DRK_EINVAL, "%s: depth compositing is not differentiated", fn);
     if (k->scene->leafx[1] == 0) return set_error(DRK_EINVAL, "%s: scene has no FOG bricks", fn);
+    if (k->scene->leafx[0] != 0)
+        return set_error(DRK_EINVAL, "%s: scenes with SDF leaves are not differentiated", fn);
     AdjointArgs A;
     A.adj_l = d_adj_lights;
     A.adj_e = d_adj_env;
@@ -750,7 +755,7 @@ int drk_kernel_footprint(const drk_kernel *k, size_t *bytes) {
 int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
     if (!s || !bytes) return set_error(DRK_EINVAL, "drk_scene_footprint: NULL argument");
     *bytes = (size_t)s->node_count * (sizeof(drk_octree_node) + 8 * sizeof(uint32_t)) +
-             (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
+             ((size_t)s->leafx[0] + s->leafx[1]) * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
     if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
     return DRK_OK;
 }

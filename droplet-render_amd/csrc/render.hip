@@ -127,9 +127,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     rng.s0 = rng.s1 = rng.s2 = rng.s3 = 1u;
     // Woodcock state of the current leaf.
     float4 ce = make_float4(0, 0, 0, 1);
-    const float *brick = nullptr;
-    size_t brick_off = 0;
-    uint32_t leaf_brick = 0;
+    uint32_t leaf_brick = 0;              // FOG brick of the current leaf (DRK_NO_BRICK: none)
+    uint32_t leaf_sdf = DRK_NO_BRICK;     // SDF brick of the current leaf
     float smax = 1.0f, kmaj = 1.0f, lo = 0.0f, s0 = 0.0f, tr0 = 0.0f, tr1 = 0.0f, trb = 0.0f, sr = 0.0f, td = 0.0f;
     bool first_step = true;
     uint32_t cnt_nodes = 0, cnt_lookups = 0;
@@ -265,7 +264,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             }
         }
         if (!queued) {
-            float *g = grad_copy + brick_off;
+            float *g = grad_copy + (size_t)leaf_brick * P.lvoxc3;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 uint32_t idx;
@@ -360,19 +359,37 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             } else {
                 const drk_octree_node &n = P.ob[leaf];
                 ce = *reinterpret_cast<const float4 *>(n.ce);
-                smax = n.qval[DRK_VOLUME_FOG];
                 leaf_brick = n.volx[DRK_VOLUME_FOG];
-                brick_off = (size_t)leaf_brick * P.lvoxc3;
-                brick = P.pvol_fog + brick_off;
+                leaf_sdf = n.volx[DRK_VOLUME_SDF];
                 if (kCount) cnt_nodes += 1;
-                // Fog-only leaf: always skip to the leaf entry (src/kernel.cpp:233-269).
+                // Majorant and empty-space skip, src/kernel.cpp:233-269: a leaf
+                // with an SDF brick has majorant 1 and is entered at its entry
+                // point only when that point lies ahead and outside the surface;
+                // a fog-only leaf is always entered at its entry point (even one
+                // behind the ray's position).
+                bool skip = true;
+                if (leaf_sdf != DRK_NO_BRICK) {
+                    smax = 1.0f;
+                    skip = false;
+                    if (tra > td) {
+                        F3 r0 = mk3(ro.x + rd.x * tra, ro.y + rd.y * tra, ro.z + rd.z * tra);
+                        skip = brick_lookup(P.pvol_sdf + (size_t)leaf_sdf * P.lvoxc3, r0, ce, P.lvoxc) > 0.0f;
+                        if (kCount) cnt_lookups += 1;
+                    }
+                } else {
+                    smax = n.qval[DRK_VOLUME_FOG];
+                }
                 lo = td;
                 tr0 = tra - td;
                 tr1 = trbn - td;
                 trb = trbn;
-                td = tra;
                 seg_a = tra;
-                s0 = tr0;
+                if (skip) {
+                    td = tra;
+                    s0 = tr0;
+                } else {
+                    s0 = 0.0f;
+                }
                 kmaj = sig_e * smax;
                 first_step = true;
                 st = ST_LEAF;
@@ -395,10 +412,22 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 st = ST_END;
             } else {
                 F3 rc = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
-                float p = brick_lookup(brick, rc, ce, P.lvoxc);
-                if (kCount) cnt_lookups += 1;
+                // Inside the surface (distance <= 0) is a collision; outside,
+                // the fog density if the leaf has fog, else none (-1)
+                // (src/kernel.cpp:292-315).
+                float d = 1.0f, p;
+                if (leaf_sdf != DRK_NO_BRICK) {
+                    d = brick_lookup(P.pvol_sdf + (size_t)leaf_sdf * P.lvoxc3, rc, ce, P.lvoxc);
+                    p = (d > 0.0f && leaf_brick != DRK_NO_BRICK)
+                            ? brick_lookup(P.pvol_fog + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc)
+                            : -1.0f;
+                    if (kCount) cnt_lookups += (d > 0.0f && leaf_brick != DRK_NO_BRICK) ? 2 : 1;
+                } else {
+                    p = brick_lookup(P.pvol_fog + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc);
+                    if (kCount) cnt_lookups += 1;
+                }
                 float q = rng.uniform();
-                if (!(p / smax < q)) {
+                if (!(p / smax < q) || !(d > 0.0f)) {
                     hit = true;
                     st = ST_END;
                     if (kAdj && pass == 1) {

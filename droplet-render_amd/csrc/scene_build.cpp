@@ -61,7 +61,9 @@ static uint64_t vdb_leaf_key(int x, int y, int z) {
     return k;
 }
 
-static void collect_leaves(const float *g, uint32_t nx, uint32_t ny, uint32_t nz,
+// Active voxels: non-zero density for FOG; the narrow band |d| < band for a
+// level set (OpenVDB keeps only those voxels active, src/scene.cpp:677-681).
+static void collect_leaves(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, float band,
                            std::vector<VdbLeaf> &out) {
     const uint32_t bx = (nx + 7) / 8, by = (ny + 7) / 8, bz = (nz + 7) / 8;
     for (uint32_t kz = 0; kz < bz; ++kz)
@@ -74,7 +76,8 @@ static void collect_leaves(const float *g, uint32_t nx, uint32_t ny, uint32_t nz
                 for (uint32_t z = kz * 8; z < std::min(nz, kz * 8 + 8); ++z)
                     for (uint32_t y = ky * 8; y < std::min(ny, ky * 8 + 8); ++y)
                         for (uint32_t x = kx * 8; x < std::min(nx, kx * 8 + 8); ++x) {
-                            if (g[((size_t)z * ny + y) * nx + x] == 0.0f) continue;
+                            const float v = g[((size_t)z * ny + y) * nx + x];
+                            if (band > 0.0f ? !(std::fabs(v) < band) : v == 0.0f) continue;
                             any = true;
                             int c[3] = {(int)x, (int)y, (int)z};
                             for (int a = 0; a < 3; ++a) {
@@ -157,17 +160,23 @@ struct Builder {
     }
 };
 
-// Trilinear sample of a dense grid with background 0, restating OpenVDB's
-// BoxSampler (index = world/s in double, floor, interpolate z then y then x
-// with float differences and double weights). OpenVDB itself is not
-// available; see the oracle for the independent restatement.
-static inline float grid_at(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, long x, long y, long z) {
-    if (x < 0 || y < 0 || z < 0 || x >= (long)nx || y >= (long)ny || z >= (long)nz) return 0.0f;
-    return g[((size_t)z * ny + (size_t)y) * nx + (size_t)x];
+// Trilinear sample of a dense grid, restating OpenVDB's BoxSampler (index =
+// world/s in double, floor, interpolate z then y then x with float
+// differences and double weights). Outside the grid the value is the
+// background: 0 for FOG, +band for a level set, whose values are also held
+// to [-band, band] (OpenVDB stores +-background beyond the narrow band).
+// OpenVDB itself is not available; see the oracle for the independent
+// restatement.
+static inline float grid_at(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, float band, long x, long y,
+                            long z) {
+    if (x < 0 || y < 0 || z < 0 || x >= (long)nx || y >= (long)ny || z >= (long)nz) return band;
+    const float v = g[((size_t)z * ny + (size_t)y) * nx + (size_t)x];
+    return band > 0.0f ? std::max(std::min(v, band), -band) : v;
 }
 static inline float lerp_vdb(float a, float b, double w) { return a + (float)((double)(b - a) * w); }
 
-static float box_sample(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, double inv_s, const float p[3]) {
+static float box_sample(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, float band, double inv_s,
+                        const float p[3]) {
     double q[3];
     long i0[3];
     double uvw[3];
@@ -179,7 +188,7 @@ static float box_sample(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, d
     float d[2][2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
-            for (int k = 0; k < 2; ++k) d[i][j][k] = grid_at(g, nx, ny, nz, i0[0] + i, i0[1] + j, i0[2] + k);
+            for (int k = 0; k < 2; ++k) d[i][j][k] = grid_at(g, nx, ny, nz, band, i0[0] + i, i0[1] + j, i0[2] + k);
     return lerp_vdb(lerp_vdb(lerp_vdb(d[0][0][0], d[0][0][1], uvw[2]), lerp_vdb(d[0][1][0], d[0][1][1], uvw[2]), uvw[1]),
                     lerp_vdb(lerp_vdb(d[1][0][0], d[1][0][1], uvw[2]), lerp_vdb(d[1][1][0], d[1][1][1], uvw[2]), uvw[1]),
                     uvw[0]);
@@ -189,16 +198,22 @@ static float box_sample(const float *g, uint32_t nx, uint32_t ny, uint32_t nz, d
 
 int scene_build_dense(const float *fog, const float *sdf, uint32_t nx, uint32_t ny, uint32_t nz,
                       float s, uint32_t maxd, drk_host_scene **out) {
-    if (!fog || !out || nx == 0 || ny == 0 || nz == 0 || !(s > 0.0f) || maxd == 0 || maxd > kMaxDepth)
+    if ((!fog && !sdf) || !out || nx == 0 || ny == 0 || nz == 0 || !(s > 0.0f) || maxd == 0 || maxd > kMaxDepth)
         return set_error(DRK_EINVAL, "drk_scene_build_dense: invalid argument");
-    if (sdf) return set_error(DRK_EINVAL, "drk_scene_build_dense: SDF grids are not supported yet");
     *out = nullptr;
+    // Narrow band of the level set: bvc = 4 voxels (src/scene.cpp:844).
+    const float band = 4.0f * s;
 
     // Leaf AABBs in buffer order SDF then FOG (src/scene.cpp:676-701).
     std::vector<Aabb> boxes[2];
-    {
+    if (sdf) {
         std::vector<VdbLeaf> leaves;
-        collect_leaves(fog, nx, ny, nz, leaves);
+        collect_leaves(sdf, nx, ny, nz, band, leaves);
+        for (const VdbLeaf &L : leaves) boxes[DRK_VOLUME_SDF].push_back(leaf_aabb(L, s));
+    }
+    if (fog) {
+        std::vector<VdbLeaf> leaves;
+        collect_leaves(fog, nx, ny, nz, 0.0f, leaves);
         for (const VdbLeaf &L : leaves) boxes[DRK_VOLUME_FOG].push_back(leaf_aabb(L, s));
     }
     if (boxes[0].empty() && boxes[1].empty()) return set_error(DRK_EINVAL, "drk_scene_build_dense: grid has no active voxels");
@@ -256,6 +271,11 @@ int scene_build_dense(const float *fog, const float *sdf, uint32_t nx, uint32_t 
         return set_error(DRK_ENOMEM, "drk_scene_build_dense: out of host memory");
     }
     std::memcpy(hs->ob, B.ob.data(), sizeof(drk_octree_node) * hs->node_count);
+    // The node the resampling skips (the last one, below) keeps the bricks it
+    // was given here: FOG 0 (its majorant 0 makes it transparent anyway) and
+    // SDF +band, outside any surface. The reference leaves both uninitialised.
+    if (hs->pvol[DRK_VOLUME_SDF])
+        for (size_t j = 0; j < (size_t)B.leafx[DRK_VOLUME_SDF] * lvoxc3; ++j) hs->pvol[DRK_VOLUME_SDF][j] = band;
 
     // Resampling, src/scene.cpp:870-908. The reference iterates node indices
     // [0, index) where index is the LAST assigned node index, so the final
@@ -268,24 +288,45 @@ int scene_build_dense(const float *fog, const float *sdf, uint32_t nx, uint32_t 
     auto work = [&](uint32_t i0, uint32_t i1) {
         for (uint32_t i = i0; i < i1; ++i) {
             drk_octree_node &n = hs->ob[i];
-            if (n.volx[DRK_VOLUME_SDF] == DRK_NO_BRICK && n.volx[DRK_VOLUME_FOG] == DRK_NO_BRICK) continue;
-            // With no SDF grid the sampled distance is the positive background, so
-            // no fog leaf is culled (src/scene.cpp:874-883).
+            if (n.volx[DRK_VOLUME_SDF] == DRK_NO_BRICK) {
+                if (n.volx[DRK_VOLUME_FOG] == DRK_NO_BRICK) continue;   // not a leaf
+                // A fog leaf whose centre lies inside the surface (and that no
+                // SDF leaf overlaps) is dropped (src/scene.cpp:874-883).
+                // Without an SDF grid the distance is the positive background.
+                if (sdf) {
+                    float pc[3] = {n.ce[0], n.ce[1], n.ce[2]};
+                    if (box_sample(sdf, nx, ny, nz, band, inv_s, pc) < 0.0f) {
+                        n.volx[DRK_VOLUME_FOG] = DRK_NO_BRICK;
+                        continue;
+                    }
+                }
+            }
             F4 nc = f4(n.ce[0], n.ce[1], n.ce[2], n.ce[3]);
             F4 ne = splat(n.ce[3]);
             n.qval[DRK_VOLUME_SDF] = FLT_MAX;
             n.qval[DRK_VOLUME_FOG] = 0.01f;
-            if (n.volx[DRK_VOLUME_FOG] == DRK_NO_BRICK) continue;
-            float *brick = hs->pvol[DRK_VOLUME_FOG] + (size_t)n.volx[DRK_VOLUME_FOG] * lvoxc3;
+            float *sbrick = n.volx[DRK_VOLUME_SDF] == DRK_NO_BRICK
+                                ? nullptr
+                                : hs->pvol[DRK_VOLUME_SDF] + (size_t)n.volx[DRK_VOLUME_SDF] * lvoxc3;
+            float *fbrick = n.volx[DRK_VOLUME_FOG] == DRK_NO_BRICK
+                                ? nullptr
+                                : hs->pvol[DRK_VOLUME_FOG] + (size_t)n.volx[DRK_VOLUME_FOG] * lvoxc3;
             for (uint32_t j = 0; j < lvoxc3; ++j) {
                 F4 idx = f4((float)(j % lvoxc), (float)((j / lvoxc) % lvoxc), (float)(j / (lvoxc * lvoxc)), 1.0f);
                 F4 nn = (splat(nv) - 2.0f * idx - splat(1.0f)) / (splat(nv) - splat(1.0f));
                 F4 nw = nc - ne * nn;
                 float p[3] = {nw.x, nw.y, nw.z};
-                float v = box_sample(fog, nx, ny, nz, inv_s, p);
-                v = std::max(std::min(v, 1.0f), 0.0f);
-                brick[j] = v;
-                n.qval[DRK_VOLUME_FOG] = std::max(n.qval[DRK_VOLUME_FOG], v);
+                if (sbrick) {
+                    float v = box_sample(sdf, nx, ny, nz, band, inv_s, p);
+                    sbrick[j] = v;
+                    n.qval[DRK_VOLUME_SDF] = std::min(n.qval[DRK_VOLUME_SDF], v);
+                }
+                if (fbrick) {
+                    float v = box_sample(fog, nx, ny, nz, 0.0f, inv_s, p);
+                    v = std::max(std::min(v, 1.0f), 0.0f);
+                    fbrick[j] = v;
+                    n.qval[DRK_VOLUME_FOG] = std::max(n.qval[DRK_VOLUME_FOG], v);
+                }
             }
         }
     };

@@ -13,12 +13,14 @@ from . import _lib as L
 
 
 class Scene:
-    def __init__(self, ob, pvol_fog, lvoxc, mlevel=None, root_ce=None, voxel_size=None):
+    def __init__(self, ob, pvol_fog, lvoxc, mlevel=None, root_ce=None, voxel_size=None, pvol_sdf=None):
         self.ob = np.ascontiguousarray(ob)            # structured node array (see node_dtype)
-        self.pvol = [None, np.ascontiguousarray(pvol_fog, dtype=np.float32)]
+        self.pvol = [None if pvol_sdf is None else np.ascontiguousarray(pvol_sdf, dtype=np.float32),
+                     np.ascontiguousarray(pvol_fog, dtype=np.float32)]
         self.lvoxc = int(lvoxc)
         self.lvoxc3 = self.lvoxc ** 3
-        self.leafx = [0, int(self.pvol[1].size // self.lvoxc3)]
+        self.leafx = [0 if pvol_sdf is None else int(self.pvol[0].size // self.lvoxc3),
+                      int(self.pvol[1].size // self.lvoxc3)]
         self.index = len(self.ob) - 1
         self.mlevel = mlevel
         self.root_ce = root_ce
@@ -28,14 +30,18 @@ class Scene:
     node_dtype = np.dtype([("ce", "<f4", (4,)), ("chn", "<u4", (8,)), ("volx", "<u4", (2,)), ("qval", "<f4", (2,))])
 
     @classmethod
-    def from_grid(cls, fog, voxel_size=None, maxd=12):
+    def from_grid(cls, fog, voxel_size=None, maxd=12, sdf=None):
         """Builds the octree from a dense (nz, ny, nx) float grid (x fastest);
-        voxel (i,j,k) sits at world (i,j,k)*voxel_size (default 1/nx)."""
-        fog = np.ascontiguousarray(fog, dtype=np.float32)
-        nz, ny, nx = fog.shape
+        voxel (i,j,k) sits at world (i,j,k)*voxel_size (default 1/nx). `sdf`
+        (same shape, world-unit signed distance, negative inside; `fog` may
+        then be None) adds SDF leaves (drk_scene_build_dense)."""
+        ref = fog if fog is not None else sdf
+        nz, ny, nx = ref.shape
+        fog = None if fog is None else np.ascontiguousarray(fog, dtype=np.float32)
+        sdf = None if sdf is None else np.ascontiguousarray(sdf, dtype=np.float32)
         s = float(voxel_size) if voxel_size else 1.0 / nx
         out = C.POINTER(L.HostScene)()
-        L.check(L.lib().drk_scene_build_dense(L.fptr(fog), None, nx, ny, nz, s, maxd, C.byref(out)))
+        L.check(L.lib().drk_scene_build_dense(L.fptr(fog), L.fptr(sdf), nx, ny, nz, s, maxd, C.byref(out)))
         return cls._adopt(out, s)
 
     @classmethod
@@ -49,7 +55,9 @@ class Scene:
             l3 = hs.lvoxc ** 3
             nf = hs.leafx[1] * l3
             pf = (np.ctypeslib.as_array(hs.pvol[1], shape=(nf,)).copy() if nf else np.zeros(0, np.float32))
-            sc = cls(ob, pf, hs.lvoxc, hs.mlevel, np.array(hs.root_ce[:], np.float32), voxel_size)
+            ns = hs.leafx[0] * l3
+            ps = np.ctypeslib.as_array(hs.pvol[0], shape=(ns,)).copy() if ns else None
+            sc = cls(ob, pf, hs.lvoxc, hs.mlevel, np.array(hs.root_ce[:], np.float32), voxel_size, ps)
         finally:
             L.lib().drk_host_scene_free(out)
         return sc
@@ -59,9 +67,9 @@ class Scene:
         hs = L.HostScene()
         hs.ob = self.ob.ctypes.data_as(C.POINTER(L.OctreeNode))
         hs.node_count = self.node_count
-        hs.pvol[0] = None
+        hs.pvol[0] = L.fptr(self.pvol[0]) if self.leafx[0] else None
         hs.pvol[1] = L.fptr(self.pvol[1]) if self.leafx[1] else None
-        hs.leafx[0] = 0
+        hs.leafx[0] = self.leafx[0]
         hs.leafx[1] = self.leafx[1]
         hs.lvoxc = self.lvoxc
         hs.mlevel = self.mlevel or 0
@@ -92,8 +100,9 @@ class Scene:
         if self._dev is None:
             h = C.c_void_p()
             obp = self.ob.ctypes.data_as(C.POINTER(L.OctreeNode))
-            L.check(L.lib().drk_scene_create(obp, self.node_count, None, 0, L.fptr(self.pvol[1]), self.leafx[1],
-                                             self.lvoxc, C.byref(h)))
+            sdf = self.pvol[0] if self.leafx[0] else None
+            L.check(L.lib().drk_scene_create(obp, self.node_count, L.fptr(sdf), self.leafx[0], L.fptr(self.pvol[1]),
+                                             self.leafx[1], self.lvoxc, C.byref(h)))
             self._dev = h
         return self._dev
 

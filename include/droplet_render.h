@@ -57,9 +57,15 @@ typedef struct drk_host_scene {
  * Scene::Initialize: leaf-AABB collection and root sizing
  * (src/scene.cpp:676-716), Octree::BuildPath (src/scene.cpp:267-305) and the
  * brick resampling with per-leaf majorants (src/scene.cpp:840-917).
- * Grids are x-fastest, voxel (i,j,k) at world (i,j,k)*voxel_size; a voxel is
- * active when its value differs from the background 0. sdf may be NULL.
- * maxd is the depth cap (src/scene.cpp:712-716).
+ * Grids are x-fastest, voxel (i,j,k) at world (i,j,k)*voxel_size. A FOG
+ * voxel is active when its value differs from the background 0. sdf (may be
+ * NULL, as may fog, not both) is a signed distance in world units, negative
+ * inside: it stands for the reference's level set, whose active voxels are
+ * the narrow band |d| < 4*voxel_size (bvc, src/scene.cpp:844) and whose
+ * values beyond it are +-that band (also the background outside the grid).
+ * SDF leaves get SDF bricks and qval[SDF] = their minimum distance; FOG-only
+ * leaves whose centre is inside the surface lose their FOG brick
+ * (src/scene.cpp:874-883). maxd is the depth cap (src/scene.cpp:712-716).
  */
 int drk_scene_build_dense(const float *fog, const float *sdf,
                           uint32_t nx, uint32_t ny, uint32_t nz,

@@ -483,6 +483,7 @@ static void map_env(const float *tex, uint32_t w, uint32_t h, const float rd[3],
 typedef struct {
     const onode *ob;
     const float *pvol_fog;
+    const float *pvol_sdf;   /* NULL without SDF leaves */
     uint32_t lvoxc, lvoxc3;
     float viewi[16], proji[16];
     uint32_t w, h;
@@ -535,11 +536,26 @@ static int track(octx *X, const float ro[3], const float rd[3], uint32_t ray_id,
         float tra = lf->t0, trb = lf->t1;
         float lo = td;
         float tr0 = tra - td, tr1 = trb - td;
-        float smax = N->qval[1];
-        /* sole fog leaf: sm = true, skip to the leaf entry */
-        td = tra;
-        float s0 = tr0;
-        const float *brick = P->pvol_fog + (size_t)N->volx[1] * P->lvoxc3;
+        const int sdf = N->volx[0] != 0xFFFFFFFFu;
+        const float *fogb = N->volx[1] != 0xFFFFFFFFu ? P->pvol_fog + (size_t)N->volx[1] * P->lvoxc3 : NULL;
+        const float *sdfb = sdf ? P->pvol_sdf + (size_t)N->volx[0] * P->lvoxc3 : NULL;
+        /* kernel.cpp:245-269: an SDF leaf has majorant 1 and is entered at
+           its entry point only when that lies ahead (tra > td) and outside
+           the surface (distance there > 0); a sole fog leaf always is. */
+        float smax = sdf ? 1.0f : N->qval[1];
+        int skip = 1;
+        if (sdf) {
+            skip = 0;
+            if (tra > td) {
+                float r0[3] = {ro[0] + rd[0] * tra, ro[1] + rd[1] * tra, ro[2] + rd[2] * tra};
+                skip = or_voxel(sdfb, r0, N->ce, P->lvoxc) > 0.0f;
+            }
+        }
+        float s0 = 0.0f;
+        if (skip) {
+            td = tra;
+            s0 = tr0;
+        }
         float sr = -(logf(stream_u(&rng)) / (sige * smax));
         int sm = 1;
         for (;;) {
@@ -551,9 +567,17 @@ static int track(octx *X, const float ro[3], const float rd[3], uint32_t ray_id,
             if (sm) rm = sc > tr0;
             if (sm && rm) {
                 float rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
-                float p = or_voxel(brick, rc, N->ce, P->lvoxc);
+                /* kernel.cpp:292-315: inside the surface is a collision;
+                   outside, the fog if any (else -1: never accepted). */
+                float d = 1.0f, p;
+                if (sdf) {
+                    d = or_voxel(sdfb, rc, N->ce, P->lvoxc);
+                    p = (d > 0.0f && fogb) ? or_voxel(fogb, rc, N->ce, P->lvoxc) : -1.0f;
+                } else {
+                    p = or_voxel(fogb, rc, N->ce, P->lvoxc);
+                }
                 float q = stream_u(&rng);
-                rm = p / smax < q;
+                rm = (p / smax < q) && (d > 0.0f);
                 if (rm) sr = sr - logf(stream_u(&rng)) / (sige * smax);
             }
         }
@@ -920,6 +944,7 @@ static void sample_volume_adj(octx *X, oadj *A, const float ro[3], const float r
  */
 int or_adjoint(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                uint32_t sample_base, const float *adj, float *grad, size_t grad_n) {
+    if (P->pvol_sdf) return -1;   /* SDF leaves are not differentiated */
     memset(grad, 0, sizeof(float) * grad_n);
     size_t plane = (size_t)tilex * tiley * 4;
     uint32_t rx = tilex & ~1u, ry = tiley & ~1u;

@@ -39,7 +39,7 @@ class OMie(C.Structure):
 
 
 class OParams(C.Structure):
-    _fields_ = [("ob", C.c_void_p), ("pvol_fog", _F), ("lvoxc", C.c_uint32), ("lvoxc3", C.c_uint32),
+    _fields_ = [("ob", C.c_void_p), ("pvol_fog", _F), ("pvol_sdf", _F), ("lvoxc", C.c_uint32), ("lvoxc3", C.c_uint32),
                 ("viewi", C.c_float * 16), ("proji", C.c_float * 16), ("w", C.c_uint32), ("h", C.c_uint32),
                 ("scattevs", C.c_uint32), ("sigma_s", C.c_float), ("sigma_a", C.c_float), ("phase", C.c_uint32),
                 ("hg_g", C.c_float), ("env", C.c_uint32), ("env_map", _F), ("env_w", C.c_uint32),
@@ -217,6 +217,11 @@ def make_params(wl, seed, depth=None):
     keep = [np.ascontiguousarray(sc.ob), f32(sc.pvol[1])]
     P.ob = keep[0].ctypes.data
     P.pvol_fog = fp(keep[1])
+    if sc.pvol[0] is not None and sc.leafx[0]:
+        keep.append(f32(sc.pvol[0]))
+        P.pvol_sdf = fp(keep[-1])
+    else:
+        P.pvol_sdf = None
     P.lvoxc, P.lvoxc3 = sc.lvoxc, sc.lvoxc ** 3
     P.viewi[:] = [float(x) for x in f32(wl.viewi).reshape(-1)]
     P.proji[:] = [float(x) for x in f32(wl.proji).reshape(-1)]
@@ -307,16 +312,17 @@ def _f(x):
     return np.float32(x)
 
 
-def vdb_leaves(grid):
+def vdb_leaves(grid, band=0.0):
     """Active 8^3 leaves of a dense (nz,ny,nx) grid in OpenVDB tree order with
-    their active index bounding boxes (min, max inclusive)."""
+    their active index bounding boxes (min, max inclusive). Active: non-zero,
+    or |value| < band for a level set's narrow band."""
     nz, ny, nx = grid.shape
     out = []
     for bz in range(0, nz, 8):
         for by in range(0, ny, 8):
             for bx in range(0, nx, 8):
                 blk = grid[bz:bz + 8, by:by + 8, bx:bx + 8]
-                nzi = np.nonzero(blk)
+                nzi = np.nonzero(np.abs(blk) < np.float32(band)) if band > 0 else np.nonzero(blk)
                 if len(nzi[0]) == 0:
                     continue
                 mn = (bx + nzi[2].min(), by + nzi[1].min(), bz + nzi[0].min())
@@ -328,23 +334,35 @@ def vdb_leaves(grid):
     return [(mn, mx) for _, mn, mx in out]
 
 
-def build_octree(grid, s=None, maxd=12):
+def build_octree(grid, s=None, maxd=12, sdf=None):
     """Pure-Python restatement of the octree build: leaf AABBs
-    (scene.cpp:676-701), root sizing (:703-716), BuildPath (:267-305) and the
-    resampling with majorants (:870-908), with float32 arithmetic where the
-    reference uses float. Returns (nodes list of dicts, bricks (leafx,512), lvoxc, mlevel)."""
-    grid = np.asarray(grid, dtype=np.float32)
-    nz, ny, nx = grid.shape
+    (scene.cpp:676-701, buffer order SDF then FOG), root sizing (:703-716),
+    BuildPath (:267-305) and the resampling with majorants and the culling of
+    fog leaves inside the surface (:870-908), with float32 arithmetic where
+    the reference uses float. `sdf`: a signed-distance grid standing for the
+    level set (narrow band 4 voxels, values held to +-band, band outside).
+    Returns (nodes list of dicts, FOG bricks (leafx,lvoxc^3), lvoxc, mlevel),
+    plus the SDF bricks when `sdf` is given."""
+    ref = grid if grid is not None else sdf
+    nz, ny, nx = np.asarray(ref).shape
+    grid = None if grid is None else np.asarray(grid, dtype=np.float32)
+    sdf = None if sdf is None else np.asarray(sdf, dtype=np.float32)
     s = _f(s if s else 1.0 / nx)
-    boxes = []
-    for mn, mx in vdb_leaves(grid):
-        c, e = [], []
-        for a in range(3):
-            dimi = _f(mx[a] - mn[a] + 1)
-            e.append(_f(_f(_f(0.5) * dimi) * s))
-            posi = (float(mn[a]) + float(mx[a])) * 0.5
-            c.append(_f(posi * float(s)))
-        boxes.append((c, e))
+    band = _f(_f(4.0) * s)
+
+    def aabbs(g, bnd):
+        out = []
+        for mn, mx in vdb_leaves(g, bnd):
+            c, e = [], []
+            for a in range(3):
+                dimi = _f(mx[a] - mn[a] + 1)
+                e.append(_f(_f(_f(0.5) * dimi) * s))
+                posi = (float(mn[a]) + float(mx[a])) * 0.5
+                c.append(_f(posi * float(s)))
+            out.append((c, e))
+        return out
+    boxes_b = [aabbs(sdf, float(band)) if sdf is not None else [], aabbs(grid, 0.0) if grid is not None else []]
+    boxes = boxes_b[0] + boxes_b[1]
     lo = [_f(np.finfo(np.float32).max)] * 3
     hi = [_f(-np.finfo(np.float32).max)] * 3
     for c, e in boxes:
@@ -390,21 +408,25 @@ def build_octree(grid, s=None, maxd=12):
                 nodes[x]["chn"][i] = len(nodes) - 1
             build(nodes[x]["chn"][i], cc, ee, c1, e1, level + 1, bx)
 
-    for c1, e1 in boxes:
-        build(0, cr, amax, c1, e1, 0, 1)
+    for bx in range(2):
+        for c1, e1 in boxes_b[bx]:
+            build(0, cr, amax, c1, e1, 0, bx)
     lvoxc = int(lvc)
     bricks = np.zeros((leafx[1], lvoxc ** 3), np.float32)
+    sbricks = np.full((leafx[0], lvoxc ** 3), band, np.float32)   # the unresampled last node: outside
     inv_s = 1.0 / float(s)
 
-    def box_sample(p):
+    def box_sample(p, src=None, bnd=None):
+        src = grid if src is None else src
         q = [float(p[a]) * inv_s for a in range(3)]
         i0 = [math.floor(v) for v in q]
         uvw = [q[a] - i0[a] for a in range(3)]
 
         def g(x, y, z):
             if x < 0 or y < 0 or z < 0 or x >= nx or y >= ny or z >= nz:
-                return _f(0.0)
-            return grid[z, y, x]
+                return bnd if bnd is not None else _f(0.0)
+            v = src[z, y, x]
+            return max(min(v, bnd), _f(-bnd)) if bnd is not None else v
 
         def lp(a, b, w):
             return _f(a + _f(float(_f(b - a)) * w))
@@ -413,20 +435,33 @@ def build_octree(grid, s=None, maxd=12):
                   lp(lp(d_[1][0][0], d_[1][0][1], uvw[2]), lp(d_[1][1][0], d_[1][1][1], uvw[2]), uvw[1]), uvw[0])
 
     nv = _f(lvoxc)
+    NB = 0xFFFFFFFF
     for i in range(len(nodes) - 1):      # the reference resamples nodes [0, index) only
         n = nodes[i]
-        if n["volx"][1] == 0xFFFFFFFF:
-            continue
+        if n["volx"][0] == NB:
+            if n["volx"][1] == NB:
+                continue
+            if sdf is not None and box_sample(n["ce"][:3], sdf, band) < 0.0:
+                n["volx"][1] = NB         # fog leaf inside the surface: dropped
+                continue
         n["qval"] = [float(np.finfo(np.float32).max), 0.01]
         qf = _f(0.01)
+        qs = _f(np.finfo(np.float32).max)
         for j in range(lvoxc ** 3):
             idx = (j % lvoxc, (j // lvoxc) % lvoxc, j // (lvoxc * lvoxc))
             p = []
             for a in range(3):
                 nn = _f(_f(_f(nv - _f(_f(2.0) * _f(idx[a]))) - _f(1.0)) / _f(nv - _f(1.0)))
                 p.append(_f(n["ce"][a] - _f(n["ce"][3] * nn)))
-            v = max(min(box_sample(p), _f(1.0)), _f(0.0))
-            bricks[n["volx"][1], j] = v
-            qf = max(qf, v)
-        n["qval"][1] = float(qf)
+            if n["volx"][0] != NB:
+                v = box_sample(p, sdf, band)
+                sbricks[n["volx"][0], j] = v
+                qs = min(qs, v)
+            if n["volx"][1] != NB:
+                v = max(min(box_sample(p), _f(1.0)), _f(0.0))
+                bricks[n["volx"][1], j] = v
+                qf = max(qf, v)
+        n["qval"] = [float(qs), float(qf)]
+    if sdf is not None:
+        return nodes, bricks, lvoxc, mlevel, sbricks
     return nodes, bricks, lvoxc, mlevel

@@ -136,3 +136,49 @@ def test_brick_vertices_sample_the_grid(pkg):
         expect = 0.2 + 0.5 * vx
         got = b[4, 4, :]
         assert np.allclose(got[ok], expect[ok], atol=2e-6), (i, got, expect)
+
+
+def sphere_sdf(n, c=(0.5, 0.5, 0.5), r=0.25):
+    """Signed distance (world units, negative inside) of a sphere on an n^3 grid, voxel (i,j,k) at (i,j,k)/n."""
+    z, y, x = np.mgrid[0:n, 0:n, 0:n].astype(np.float64) / n
+    return (np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - r).astype(np.float32)
+
+
+def test_sdf_build_matches_python_restatement(pkg, oracle):
+    """SDF leaves (narrow band 4 voxels) before FOG leaves, SDF bricks and
+    qval[SDF] = min distance, and fog leaves inside the surface dropped
+    (src/scene.cpp:676-701, 874-904): C++ against the Python restatement."""
+    n = 128
+    sdf = sphere_sdf(n, r=0.42)
+    fog = pkg.configs.synth_fbm(n, 5, octaves=3, freq=3.0, threshold=-1.0)   # fog everywhere
+    sc = pkg.Scene.from_grid(fog, sdf=sdf)
+    nodes, bricks, lvoxc, mlevel, sbricks = oracle.build_octree(fog, sdf=sdf)
+    assert (sc.lvoxc, sc.mlevel) == (lvoxc, mlevel) and len(nodes) == sc.node_count
+    for i, nd in enumerate(nodes):
+        assert np.array_equal(np.float32(nd["ce"]), sc.ob["ce"][i]), i
+        assert list(sc.ob["chn"][i]) == nd["chn"], i
+        assert list(sc.ob["volx"][i]) == nd["volx"], i
+        assert np.array_equal(np.float32(nd["qval"]), sc.ob["qval"][i]), i
+    assert np.array_equal(bricks.reshape(-1), sc.pvol[1])
+    assert np.array_equal(sbricks.reshape(-1), sc.pvol[0])
+    # structure: some leaves carry both bricks, some fog leaves were dropped
+    both = (sc.ob["volx"][:, 0] != NO) & (sc.ob["volx"][:, 1] != NO)
+    assert sc.leafx[0] > 0 and both.any()
+    assert (sc.ob["volx"][:, 1] != NO).sum() < sc.leafx[1]
+
+
+def test_sdf_bricks_hold_the_clamped_distance(pkg):
+    """At brick vertices that coincide with grid points the SDF brick holds the
+    grid's distance held to the band; qval[SDF] is the brick minimum."""
+    n = 32
+    sdf = sphere_sdf(n, r=0.3)
+    sc = pkg.Scene.from_grid(None, sdf=sdf)
+    band = np.float32(4.0 / n)
+    assert sc.leafx[1] == 0 and sc.leafx[0] > 0
+    sb = sc.pvol[0].reshape(sc.leafx[0], -1)
+    assert sb.min() >= -band and sb.max() <= band
+    leaves = np.nonzero(sc.ob["volx"][:, 0] != NO)[0]
+    for i in leaves[:-1]:
+        if i == sc.node_count - 1:
+            continue
+        assert np.float32(sc.ob["qval"][i, 0]) == sb[sc.ob["volx"][i, 0]].min()

@@ -162,17 +162,22 @@ struct Traversal {
 
 constexpr uint32_t kStkNodeMask = 0x00ffffffu;
 
-// Advances to the next leaf in ray order. Returns false when exhausted.
+enum : int { kTravExhausted = 0, kTravLeaf = 1, kTravPending = 2 };
+
+// Advances the walk to the next leaf in ray order (kTravLeaf: its index and
+// [tra, trb] interval), to its end (kTravExhausted), or by at most `budget`
+// node steps (kTravPending: call again to resume where it stopped).
 // `touches` counts node-record reads (for the algorithmic-bytes model).
-__device__ __forceinline__ bool trav_next(const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk, uint32_t stride,
-                                          uint32_t &leaf, float &tra, float &trb, uint32_t &touches) {
-    for (;;) {
-        if (T.lvl < 0) return false;
+__device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk, uint32_t stride,
+                                         uint32_t &leaf, float &tra, float &trb, uint32_t &touches, uint32_t budget) {
+    for (uint32_t steps = 0;; ++steps) {
+        if (steps >= budget) return kTravPending;
+        if (T.lvl < 0) return kTravExhausted;
         uint32_t f = stk[T.lvl * stride];
         uint32_t tn = (f >> 24) & 15u;
         if (tn >= 8) {
             // Pop; recompute the parent's interval from the root along the path.
-            if (--T.lvl < 0) return false;
+            if (--T.lvl < 0) return kTravExhausted;
             F3 a0 = T.rt0, a1 = T.rt1;
             for (int k = 0; k < T.lvl; ++k) {
                 uint32_t ck = (stk[k * stride] >> 28) & 7u;
@@ -197,7 +202,7 @@ __device__ __forceinline__ bool trav_next(const uint32_t *__restrict__ chx, Trav
             leaf = ch;
             tra = fmaxf(fmaxf(ct0.x, ct0.y), ct0.z);
             trb = fminf(fminf(ct1.x, ct1.y), ct1.z);
-            return true;
+            return kTravLeaf;
         }
         // Interior child: push.
         F3 tm;
@@ -364,9 +369,19 @@ __device__ __forceinline__ F3 mie_sample(const MieLds &T, F3 iv, float u1, float
     return basis_dir(b1, b2, iv, st, ct, ph);
 }
 
+// x^1.5 as x * sqrt(x): two correctly rounded operations, so within about
+// 2^-23 relative of the exact power (the general powf costs ~20x more).
+__device__ __forceinline__ float pow_1p5(float x) { return x * sqrtf(x); }
+
+// x^2.2 as exp2(2.2 * log2(x)) on the hardware base-2 logarithm and
+// exponential: the error of log2 is scaled by 2.2 * ln 2 into the result,
+// a few 1e-7 relative for radiance-sized x; 0, infinities and NaNs map as
+// powf maps them.
+__device__ __forceinline__ float pow_2p2(float x) { return exp2f(2.2f * log2f(x)); }
+
 // HGPhase (KernelSampler.cpp:165-204).
 __device__ __forceinline__ float hg_eval(float g, float ct) {
-    return (1.0f - g * g) / (4.0f * kPi * powf(1.0f + g * g - 2.0f * g * ct, 1.5f));
+    return (1.0f - g * g) / (4.0f * kPi * pow_1p5(1.0f + g * g - 2.0f * g * ct));
 }
 __device__ __forceinline__ F3 hg_sample(float g, F3 iv, float u1, float u2) {
     float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
@@ -415,10 +430,10 @@ __device__ __forceinline__ F3 hosek_sky(const SkyState &S, F3 rd) {
     for (int i = 0; i < 3; ++i) {
         const float *c = S.cfg[i];
         float expm = expf(c[4] * gmma);
-        float miem = (1.0f + raym) / powf(1.0f + c[8] * c[8] - 2.0f * c[8] * gacs, 1.5f);
+        float miem = (1.0f + raym) / pow_1p5(1.0f + c[8] * c[8] - 2.0f * c[8] * gacs);
         float ca = (1.0f + c[0] * expf(c[1] / (thcs + 0.01f))) * (c[2] + c[3] * expm + c[5] * raym + c[6] * miem + c[7] * zenh);
         ca *= S.rad[i];
-        out[i] = 1e-3f * powf(ca, 2.2f);
+        out[i] = 1e-3f * pow_2p2(ca);
     }
     return mk3(out[0], out[1], out[2]);
 }
@@ -444,7 +459,7 @@ __device__ __forceinline__ F3 map_env(const float *__restrict__ tex, uint32_t w,
         float db = tex[((ti + 1) * w + pj + 1) * 4 + i];
         float l0 = tht * (cb - ca) + ca;
         float l1 = tht * (db - da) + da;
-        out[i] = powf(pht * (l1 - l0) + l0, 2.2f);
+        out[i] = pow_2p2(pht * (l1 - l0) + l0);
     }
     return mk3(out[0], out[1], out[2]);
 }

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
             T.lvl = 0;
             uint32_t leaf;
             float a0, a1;
-            while (trav_next(chx, T, stk, kProbeBlock, leaf, a0, a1, touches)) {
+            while (trav_next(chx, T, stk, kProbeBlock, leaf, a0, a1, touches, ~0u) == kTravLeaf) {
                 if (cnt < max_leaves) {
                     nodes[(size_t)i * max_leaves + cnt] = leaf;
                     t0o[(size_t)i * max_leaves + cnt] = a0;

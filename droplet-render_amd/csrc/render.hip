@@ -2,8 +2,11 @@
 //
 // k_render is a persistent path tracer: every lane owns one path at a time
 // and runs a flat state machine (find next leaf -> one Woodcock step -> ray
-// end / scattering event), so lanes never wait for each other at leaf, ray
-// or sample boundaries. A lane that finishes its work unit (one pixel x a
+// end / scattering event), so lanes do not wait for each other at leaf, ray
+// or sample boundaries; the one deliberate wait gathers lanes whose ray has
+// ended until enough of them run the ray-end code together (kEndBatch),
+// and new paths and ray starts each run as one converged pass per round.
+// A lane that finishes its work unit (one pixel x a
 // chunk of samples) refills from a unit counter with one wave-aggregated
 // atomic (ballot + mbcnt prefix), which keeps the wave's lanes busy until the
 // whole tile is drained. The tile's pixels are split into one row band per
@@ -38,7 +41,26 @@ namespace drk {
 
 constexpr int kBlock = 256;
 
-enum : uint32_t { ST_WORK = 0, ST_TRAV = 1, ST_LEAF = 2, ST_END = 3, ST_DONE = 4 };
+// Lane states: needs a work unit, needs its next camera path, needs its
+// pending ray started, traversing to the next leaf, tracking inside a leaf,
+// ray finished (escape, collision or scattering event pending), drained.
+enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4, ST_END = 5, ST_DONE = 6 };
+
+// Scheduling of the state machine. A traversal call runs at most
+// kTravBudget node steps before the wave moves on (a lane with a longer walk
+// resumes it in the next round instead of holding up the whole wave), and
+// the ray-end phase, which carries the expensive scattering, sky and camera
+// code, runs only once kEndBatch lanes are waiting for it or no lane can
+// make progress otherwise. Neither changes what any lane computes, only
+// when; both raise the share of active lanes per instruction.
+#ifndef DRK_TRAV_BUDGET
+#define DRK_TRAV_BUDGET 4
+#endif
+#ifndef DRK_END_BATCH
+#define DRK_END_BATCH 32
+#endif
+constexpr uint32_t kTravBudget = DRK_TRAV_BUDGET;
+constexpr uint32_t kEndBatch = DRK_END_BATCH;
 enum : uint32_t { RK_PRIMARY = 0, RK_CONT = 1, RK_SHADOW = 2, RK_SHADOW_PASS = 3 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -106,6 +128,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
     // Work unit.
     uint32_t st = ST_WORK;
+    uint32_t ray_pend = 0;   // ray id of the pending ray start (ST_RAY)
     uint32_t chunk_cur = 0, pix_local = 0, px = 0, py = 0, key0 = 0;
     const uint32_t band0 = xcc_id();   // first band: this wave's XCD
     uint32_t band_step = 0;            // bands this wave has found drained (wave-uniform)
@@ -195,11 +218,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             rd = sun_sample(K, u01(r4.x), u01(r4.y));
             if (!gm1) { st = ST_END; hit = false; return; }   // untraced lane: unshadowed
             ro = surf;
-            start_ray(shadow_ray_id(0));
+            ray_pend = shadow_ray_id(0);
+            st = ST_RAY;
             return;
         }
         if (!gm) { st = ST_END; hit = false; return; }
-        start_ray(0);
+        ray_pend = 0;
+        st = ST_RAY;
     };
 
     // Adjoint: set up the next sample of the current pixel. With a path
@@ -328,7 +353,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         adj_begin();
                 }
                 if (s_cur < s_end) {
-                    start_path();
+                    st = ST_PATH;
                     need = false;
                 }
             }
@@ -337,11 +362,15 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         }
         if (!__ballot(st != ST_DONE)) break;
 
+        // ---- new camera paths and ray starts, one converged pass each -----
+        if (st == ST_PATH) start_path();
+        if (st == ST_RAY) start_ray(ray_pend);
+
         // ---- A: find the next leaf along the ray --------------------------
         if (st == ST_TRAV) {
             uint32_t leaf = 0;
             float tra = 0.0f, trbn = 0.0f;
-            bool found;
+            bool found, continue_trav = false;
             if (T.lvl == -2) {
                 leaf = 0;
                 tra = fmaxf(fmaxf(T.rt0.x, T.rt0.y), T.rt0.z);
@@ -349,11 +378,15 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 T.lvl = -1;
                 found = true;
             } else {
-                found = trav_next(P.chx, T, stk, stride, leaf, tra, trbn, cnt_nodes);
+                const int r = trav_next(P.chx, T, stk, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
+                if (r == kTravPending) continue_trav = true;
+                found = r == kTravLeaf;
             }
             // Compositing depth stops primary rays (src/kernel.cpp:217-222).
             if (found && kind == RK_PRIMARY && !(td < pix_depth)) found = false;
-            if (!found) {
+            if (continue_trav) {
+                // walk budget spent: stay in ST_TRAV
+            } else if (!found) {
                 st = ST_END;
                 hit = false;
             } else {
@@ -439,7 +472,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         }
 
         // ---- C: ray finished (escaped or collided) -------------------------
-        if (st == ST_END) {
+        const bool run_end = (uint32_t)__popcll(__ballot(st == ST_END)) >= kEndBatch ||
+                             !__ballot(st == ST_TRAV || st == ST_LEAF);
+        if (run_end && st == ST_END) {
             bool path_done = false;
             if (kind == RK_SHADOW_PASS) {
                 // SampleVolume at r = scattevs returns the lights' radiance when
@@ -464,7 +499,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 depth_r += 1;
                 kind = RK_CONT;
                 if (kAdj && pass == 1) wseg = ltot - collected;
-                start_ray(cont_ray_id(event));
+                ray_pend = cont_ray_id(event);
+                st = ST_RAY;
             } else if (!hit) {
                 if (depth_r > 0) {
                     F3 lc = lights_eval(K, rd);
@@ -521,7 +557,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                             collected += wseg;
                         }
                     }
-                    start_ray(shadow_ray_id(event));
+                    ray_pend = shadow_ray_id(event);
+                    st = ST_RAY;
                 }
             }
             if (path_done && kAdj) {
@@ -531,14 +568,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     // Replay the same path for the gradients. (A zero total does not
                     // mean zero sub-path weights when adjoints have mixed signs.)
                     pass = 1;
-                    start_path();
+                    st = ST_PATH;
                     path_done = false;
                 }
                 if (path_done) {
                     ++s_cur;
                     if (s_cur < s_end) {
                         adj_begin();
-                        start_path();
+                        st = ST_PATH;
                     } else {
                         st = ST_WORK;
                     }
@@ -562,7 +599,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 }
                 ++s_cur;
                 if (s_cur < s_end) {
-                    start_path();
+                    st = ST_PATH;
                 } else {
                     // Unit complete: write its sums.
                     size_t o = (size_t)chunk_cur * W.npix + pix_local;

@@ -19,6 +19,7 @@ constexpr uint32_t kMaxDepth = 16;       // traversal stack levels (reference MA
 constexpr uint32_t kMaxNodes = 1u << 24; // node index field of a traversal stack word
 constexpr uint32_t kMieBins = 256;       // miedl, KernelSampler.cpp:143
 constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.318309886183790671538f;   // 1/pi rounded to float
 
 // ---------------------------------------------------------------------------
 // Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as

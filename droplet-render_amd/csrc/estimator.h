@@ -179,6 +179,7 @@ __device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Trave
             // Pop; recompute the parent's interval from the root along the path.
             if (--T.lvl < 0) return kTravExhausted;
             F3 a0 = T.rt0, a1 = T.rt1;
+#pragma nounroll   // usually one or two levels; unrolled copies only cost issue slots
             for (int k = 0; k < T.lvl; ++k) {
                 uint32_t ck = (stk[k * stride] >> 28) & 7u;
                 F3 b0, b1;
@@ -319,7 +320,7 @@ __device__ __forceinline__ F3 basis_dir(F3 b1, F3 b2, F3 iv, float st, float ct,
 __device__ __forceinline__ F3 mie_eval_rgb(const MieLds &T, float ct) {
 #pragma clang fp contract(off)
     float ct1 = fmaxf(fminf(ct, 1.0f), -1.0f);
-    float a = acosf(-ct1) / kPi;
+    float a = acosf(-ct1) * kInvPi;
     float b = 256.0f * a;
     float c = fmaxf(fminf(floorf(b), 254.0f), 0.0f);
     float t = b - c;
@@ -369,6 +370,11 @@ __device__ __forceinline__ F3 mie_sample(const MieLds &T, F3 iv, float u1, float
     return basis_dir(b1, b2, iv, st, ct, ph);
 }
 
+// x / y through the hardware reciprocal (within about 2 ulp), for values
+// that only scale radiance (phase and sky values, path weights) and never
+// steer a ray, so they need not round exactly as the oracle does.
+__device__ __forceinline__ float div_approx(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
 // x^1.5 as x * sqrt(x): two correctly rounded operations, so within about
 // 2^-23 relative of the exact power (the general powf costs ~20x more).
 __device__ __forceinline__ float pow_1p5(float x) { return x * sqrtf(x); }
@@ -381,7 +387,7 @@ __device__ __forceinline__ float pow_2p2(float x) { return exp2f(2.2f * log2f(x)
 
 // HGPhase (KernelSampler.cpp:165-204).
 __device__ __forceinline__ float hg_eval(float g, float ct) {
-    return (1.0f - g * g) / (4.0f * kPi * pow_1p5(1.0f + g * g - 2.0f * g * ct));
+    return div_approx(1.0f - g * g, 4.0f * kPi * pow_1p5(1.0f + g * g - 2.0f * g * ct));
 }
 __device__ __forceinline__ F3 hg_sample(float g, F3 iv, float u1, float u2) {
     float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
@@ -407,8 +413,8 @@ __device__ __forceinline__ F3 hosek_sky(const SkyState &S, F3 rd) {
     float clth = S.dir[2];
     float slph = S.dir[1];
     float clph = S.dir[0];
-    float rdd = rd.x / rd.y;
-    float acr = 1.0f / sqrtf(rdd * rdd + 1.0f);
+    float rdd = div_approx(rd.x, rd.y);
+    float acr = __builtin_amdgcn_rsqf(rdd * rdd + 1.0f);
     // rd.y == +-0 makes rdd infinite and acr*(...) 0*inf (NaN): take the limit
     // from the side of the zero's sign, acr*rdd -> sign(rd.x)*sign(rd.y).
     float czp;
@@ -430,8 +436,8 @@ __device__ __forceinline__ F3 hosek_sky(const SkyState &S, F3 rd) {
     for (int i = 0; i < 3; ++i) {
         const float *c = S.cfg[i];
         float expm = expf(c[4] * gmma);
-        float miem = (1.0f + raym) / pow_1p5(1.0f + c[8] * c[8] - 2.0f * c[8] * gacs);
-        float ca = (1.0f + c[0] * expf(c[1] / (thcs + 0.01f))) * (c[2] + c[3] * expm + c[5] * raym + c[6] * miem + c[7] * zenh);
+        float miem = div_approx(1.0f + raym, pow_1p5(1.0f + c[8] * c[8] - 2.0f * c[8] * gacs));
+        float ca = (1.0f + c[0] * expf(div_approx(c[1], thcs + 0.01f))) * (c[2] + c[3] * expm + c[5] * raym + c[6] * miem + c[7] * zenh);
         ca *= S.rad[i];
         out[i] = 1e-3f * pow_2p2(ca);
     }
@@ -441,11 +447,11 @@ __device__ __forceinline__ F3 hosek_sky(const SkyState &S, F3 rd) {
 // MapEnv::Evaluate (KernelSampler.cpp:369-421), followed by the pow(.,2.2)
 // of src/kernel.cpp:362-364.
 __device__ __forceinline__ F3 map_env(const float *__restrict__ tex, uint32_t w, uint32_t h, F3 rd) {
-    float tha = acosf(-rd.z) / kPi;
+    float tha = acosf(-rd.z) * kInvPi;
     float thb = (float)h * fmaxf(fminf(tha, 1.0f), 0.0f);
     float thc = fmaxf(fminf(floorf(thb), (float)(h - 2)), 0.0f);
     float tht = thb - thc;
-    float pha = 0.5f + atan2f(rd.y, -rd.x) / (2.0f * kPi);
+    float pha = 0.5f + atan2f(rd.y, -rd.x) * (0.5f * kInvPi);
     float phb = (float)w * fmaxf(fminf(pha, 1.0f), 0.0f);
     float phc = fmaxf(fminf(floorf(phb), (float)(w - 2)), 0.0f);
     float pht = phb - phc;

@@ -125,6 +125,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     const KernelConsts &K = *P.kc;
     const float sig_s = P.sigma_s;
     const float sig_e = P.sigma_e;
+    const float albedo = sig_s / sig_e;
 
     // Work unit.
     uint32_t st = ST_WORK;
@@ -481,7 +482,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 // the ray escapes (r > 0); Shadow divides by lights[0]'s colour.
                 if (!hit && P.scattevs > 0) {
                     F3 le = lights_eval(K, rd);
-                    pathL = mk3(le.x / K.sun_color[0][0], le.y / K.sun_color[0][1], le.z / K.sun_color[0][2]);
+                    pathL = mk3(div_approx(le.x, K.sun_color[0][0]), div_approx(le.y, K.sun_color[0][1]),
+                                div_approx(le.z, K.sun_color[0][2]));
                 }
                 alpha = 1.0f;
                 path_done = true;
@@ -535,15 +537,11 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         p3 = mk3(h3, h3, h3);
                     }
                     float pd1 = sun_pdf(K, srd), pd2 = sun_pdf(K, lrd);
-                    F3 w1 = mk3(p1.x / (p1.x + pd1), p1.y / (p1.y + pd1), p1.z / (p1.z + pd1));
-                    F3 w2 = mk3(p3.x / (p3.x + pd2), p3.y / (p3.y + pd2), p3.z / (p3.z + pd2));
+                    F3 w1 = mk3(div_approx(p1.x, p1.x + pd1), div_approx(p1.y, p1.y + pd1), div_approx(p1.z, p1.z + pd1));
+                    F3 w2 = mk3(div_approx(p3.x, p3.x + pd2), div_approx(p3.y, p3.y + pd2), div_approx(p3.z, p3.z + pd2));
                     F3 x = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
-                    shadow_w = mk3(beta.x * w2.x * sig_s / sig_e,
-                                   beta.y * w2.y * sig_s / sig_e,
-                                   beta.z * w2.z * sig_s / sig_e);
-                    cont_beta = mk3(beta.x * w1.x * sig_s / sig_e,
-                                    beta.y * w1.y * sig_s / sig_e,
-                                    beta.z * w1.z * sig_s / sig_e);
+                    shadow_w = mk3(beta.x * w2.x * albedo, beta.y * w2.y * albedo, beta.z * w2.z * albedo);
+                    cont_beta = mk3(beta.x * w1.x * albedo, beta.y * w1.y * albedo, beta.z * w1.z * albedo);
                     cont_dir = srd;
                     ro = x;
                     rd = lrd;

@@ -897,8 +897,20 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
     }
 }
 
+// One block per (bin, chunk of at most kEvAccChunk events): sums the chunk's
+// trilinear splats into an LDS copy of the bin's bricks, then adds the
+// non-zero sums to the gradient. The LDS sums are 64-bit fixed point: LDS
+// integer atomics run about ten times faster than ds_add_f32, and integer
+// sums do not depend on the order the lanes add in. The scale is a power of
+// two from the chunk's largest |weight| M: every splat (|weight * w| <= M,
+// w <= 1) is rounded to a multiple of 2^-S with M * 2^S < 2^45, so the at
+// most 2^17 splats of a chunk stay below 2^62, and a splat loses at most
+// 2^-46 M to rounding (float sums lose 2^-24 of the running sum per add).
+// A chunk holding a non-finite weight takes the float path so the NaN or
+// infinity reaches the gradient as before.
 __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
-    extern __shared__ float lg[];
+    extern __shared__ unsigned long long lq[];
+    __shared__ uint32_t s_max, s_bad;
     const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
     const BinTables t = bin_tables(E);
     const uint32_t total = t.blk_start[E.nbins];
@@ -915,26 +927,67 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     const uint32_t e1 = min(e0 + kEvAccChunk, t.bin_start[b + 1]);
     const uint32_t brick0 = b << E.shift;
     const uint32_t span = min(1u << E.shift, E.nbricks - brick0) * E.lvoxc3;
-    for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lg[j] = 0.0f;
+    if (threadIdx.x == 0) s_max = s_bad = 0u;
+    for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lq[j] = 0ull;
+    // Largest |weight| of the chunk (as bits: ordered like the values for
+    // non-negative floats) and whether any weight is not finite.
+    uint32_t mbits = 0u, bad = 0u;
+    for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
+        const float a = fabsf(ev.swf[i].x);
+        bad |= !(a <= 3.402823466e38f);
+        mbits = max(mbits, __float_as_uint(a));
+    }
     __syncthreads();
+    if (bad) atomicOr(&s_bad, 1u);
+    atomicMax(&s_max, mbits);
+    __syncthreads();
+    if (s_max == 0u) return;   // all weights zero: nothing to add
+    float *out = E.grad + (size_t)brick0 * E.lvoxc3;
+    if (s_bad) {
+        float *lg = reinterpret_cast<float *>(lq);
+        for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lg[j] = 0.0f;
+        __syncthreads();
+        for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
+            const float4 wf = ev.swf[i];
+            const uint2 bc = ev.sbc[i];
+            const float nl[3] = {wf.y, wf.z, wf.w};
+            float *g = lg + (bc.x - brick0) * E.lvoxc3;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                uint32_t idx;
+                float w;
+                cell_corner(bc.y, nl, E.lvoxc, v, idx, w);
+                atomicAdd(g + idx, wf.x * w);
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) {
+            const float v = lg[j];
+            if (v != 0.0f) unsafeAtomicAdd(out + j, v);
+        }
+        return;
+    }
+    const int sh = 44 - ilogbf(__uint_as_float(s_max));   // M * 2^sh < 2^45
     for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
         const float4 wf = ev.swf[i];
         const uint2 bc = ev.sbc[i];
         const float nl[3] = {wf.y, wf.z, wf.w};
-        float *g = lg + (bc.x - brick0) * E.lvoxc3;
+        unsigned long long *g = lq + (bc.x - brick0) * E.lvoxc3;
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
             uint32_t idx;
             float w;
             cell_corner(bc.y, nl, E.lvoxc, v, idx, w);
-            atomicAdd(g + idx, wf.x * w);
+            // |splat| * 2^sh < 2^45: scaled exactly, rounded to an integer
+            // exactly representable in float, converted exactly.
+            const long long q = (long long)rintf(ldexpf(wf.x * w, sh));
+            atomicAdd(g + idx, (unsigned long long)q);
         }
     }
     __syncthreads();
-    float *out = E.grad + (size_t)brick0 * E.lvoxc3;
     for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) {
-        const float v = lg[j];
-        if (v != 0.0f) unsafeAtomicAdd(out + j, v);
+        const long long q = (long long)lq[j];
+        if (q != 0) unsafeAtomicAdd(out + j, (float)ldexp((double)q, -sh));
     }
 }
 
@@ -942,7 +995,7 @@ bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nb
     shift = 0;
     while (((uint64_t)nbricks + (1ull << shift) - 1) >> shift > kEvMaxBins) ++shift;
     nbins = (uint32_t)(((uint64_t)nbricks + (1ull << shift) - 1) >> shift);
-    return nbricks > 0 && ((uint64_t)lvoxc3 * sizeof(float) << shift) <= kEvAccumLds;
+    return nbricks > 0 && ((uint64_t)lvoxc3 * sizeof(unsigned long long) << shift) <= kEvAccumLds;
 }
 
 hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
@@ -953,7 +1006,7 @@ hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
     k_ev_hist<<<sort_blocks, kEvSortBlock, E.nbins * sizeof(uint32_t), s>>>(E);
     k_ev_scan<<<1, 1024, 0, s>>>(E);
     k_ev_scatter<<<sort_blocks, kEvSortBlock, 2 * E.nbins * sizeof(uint32_t), s>>>(E);
-    const size_t lds = (size_t)E.lvoxc3 * sizeof(float) << E.shift;
+    const size_t lds = (size_t)E.lvoxc3 * sizeof(unsigned long long) << E.shift;
     k_ev_accum<<<acc_blocks, kEvAccBlock, lds, s>>>(E);
     return hipGetLastError();
 }

@@ -474,7 +474,8 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         chunk = (uint32_t)((samples + nch - 1) / nch);
     }
     chunk = std::max(1u, std::min(chunk, samples));
-    const uint32_t nchunks = (samples + chunk - 1) / chunk;
+    uint32_t nfull, nchunks;
+    plan_chunks(samples, chunk, k->chunk_override == 0, nfull, nchunks);
     if ((uint64_t)npix * nchunks > 0xffffffffull) return set_error(DRK_EINVAL, "render: too many work units");
     WorkParams W;
     std::memset(&W, 0, sizeof(W));
@@ -484,6 +485,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     W.nbands = k->bands;
     W.npix = npix;
     W.chunk = chunk;
+    W.nfull = nfull;
     W.nchunks = nchunks;
     W.total_units = npix * nchunks;
     W.out_l = d_l;
@@ -880,12 +882,14 @@ static int batch_plan(drk_batch *b) {
             }
             chunk = std::max(1u, std::min(chunk, J.samples));
         }
-        const uint32_t nchunks = (J.samples + chunk - 1) / chunk;
+        uint32_t nfull, nchunks;
+        plan_chunks(J.samples, chunk, k->chunk_override == 0, nfull, nchunks);
         if ((uint64_t)npix * nchunks > 0xffffffffull) return set_error(DRK_EINVAL, "batch: job %u has too many work units", j);
         W.counter = b->d_counters + (size_t)j * kMaxBands * kCounterStride;
         W.nbands = k->bands;
         W.npix = npix;
         W.chunk = chunk;
+        W.nfull = nfull;
         W.nchunks = nchunks;
         W.total_units = npix * nchunks;
         W.out_l = J.d_lights;

@@ -50,8 +50,8 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 // kTravBudget node steps before the wave moves on (a lane with a longer walk
 // resumes it in the next round instead of holding up the whole wave), and
 // the ray-end phase, which carries the expensive scattering, sky and camera
-// code, runs only once kEndBatch lanes are waiting for it or no lane can
-// make progress otherwise. Neither changes what any lane computes, only
+// code, runs only once kEndBatch lanes (at most half the wave's live lanes)
+// are waiting for it or no lane can make progress otherwise. Neither changes what any lane computes, only
 // when; both raise the share of active lanes per instruction.
 #ifndef DRK_TRAV_BUDGET
 #define DRK_TRAV_BUDGET 4
@@ -332,8 +332,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 px = P.x0 + lx;
                 py = P.y0 + ly;
                 key0 = py * P.w + px;
-                s_cur = chunk_cur * W.chunk;
-                s_end = min(s_cur + W.chunk, P.samples);
+                s_cur = chunk_start(W.chunk, W.nfull, P.samples, chunk_cur);
+                s_end = chunk_start(W.chunk, W.nfull, P.samples, chunk_cur + 1);
                 pix_depth = P.depth ? P.depth[key0] : 3.402823466e38f;
                 accL[0] = accL[1] = accL[2] = 0.0f;
                 accE[0] = accE[1] = accE[2] = 0.0f;
@@ -473,7 +473,12 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         }
 
         // ---- C: ray finished (escaped or collided) -------------------------
-        const bool run_end = (uint32_t)__popcll(__ballot(st == ST_END)) >= kEndBatch ||
+        // (The batch shrinks with the wave's remaining lanes, so that in the
+        // drain at the end of a launch the last paths do not wait for each
+        // other's ray ends.)
+        const uint32_t n_live = (uint32_t)__popcll(__ballot(st != ST_DONE));
+        const uint32_t end_batch = min(kEndBatch, max(1u, n_live / 2));
+        const bool run_end = (uint32_t)__popcll(__ballot(st == ST_END)) >= end_batch ||
                              !__ballot(st == ST_TRAV || st == ST_LEAF);
         if (run_end && st == ST_END) {
             bool path_done = false;

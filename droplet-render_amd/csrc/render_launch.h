@@ -12,13 +12,44 @@ namespace drk {
 // bands [b*npix/nbands, (b+1)*npix/nbands) of bp pixels; unit u of band b is
 // chunk u / bp, pixel b*npix/nbands + u % bp.
 constexpr uint32_t kMaxBands = 16;
+
+// First sample of chunk k, k <= nchunks. Chunks below nfull hold `chunk`
+// samples (the last of them fewer when the samples run out first); each
+// chunk after them takes half, rounded up, of the samples that remain, down
+// to single samples. Units are handed out chunk by chunk, so the last units
+// of a launch are short and its drain leaves few lanes idle.
+__host__ __device__ inline uint32_t chunk_start(uint32_t chunk, uint32_t nfull, uint32_t samples, uint32_t k) {
+    if (k <= nfull) return (uint32_t)((uint64_t)k * chunk < samples ? (uint64_t)k * chunk : samples);
+    uint32_t s = nfull * chunk, rem = samples - s;
+    for (uint32_t j = nfull; j < k; ++j) {
+        const uint32_t z = (rem + 1) / 2;
+        s += z;
+        rem -= z;
+    }
+    return s;
+}
+
+// Chunking of `samples` with leading chunks of `chunk` samples: fixed-size
+// chunks only (guided false), or full chunks while more than two chunks'
+// worth remain and the halving tail after them.
+inline void plan_chunks(uint32_t samples, uint32_t chunk, bool guided, uint32_t &nfull, uint32_t &nchunks) {
+    if (!guided) {
+        nfull = nchunks = (samples + chunk - 1) / chunk;
+        return;
+    }
+    const uint64_t two = 2ull * chunk;
+    nfull = samples > two ? (uint32_t)((samples - two + chunk - 1) / chunk) : 0u;
+    nchunks = nfull;
+    for (uint32_t rem = samples - nfull * chunk; rem > 0; rem -= (rem + 1) / 2) ++nchunks;
+}
 constexpr uint32_t kCounterStride = 16;   // u32 words: one 64-byte line per band counter
 struct WorkParams {
     uint32_t *counter;        // per band: next unfetched unit, kCounterStride apart (reset per launch)
     uint32_t nbands;          // row bands of the tile's pixels (1..kMaxBands)
     uint32_t total_units;     // npix * nchunks
     uint32_t npix;            // rx * ry
-    uint32_t chunk;           // samples per unit
+    uint32_t chunk;           // samples per unit in the leading chunks
+    uint32_t nfull;           // leading chunks of `chunk` samples (see chunk_start)
     uint32_t nchunks;
     float *partial;           // [buffers][nchunks][npix] float4 when nchunks > 1, else nullptr
     float *out_l, *out_e;     // tile float4 outputs (stride tilex)

@@ -85,37 +85,26 @@ __device__ __forceinline__ void camera_ray(const KernelConsts &K, uint32_t w, ui
 // operations (tm = 0.5f*(t0+t1)), so they are bit-identical to the values the
 // reference keeps on its stack.
 // ---------------------------------------------------------------------------
+// First child the ray enters (OctreeFirstNode, KernelOctree.cpp:9-34): the
+// entry plane is the largest of t0 (ties resolved as the reference's nested
+// comparisons do); a bit is set for each other axis whose midpoint is
+// crossed before the entry. Written with selects, not branches: the lanes of
+// a wave then run one path instead of each branch in turn.
 __device__ __forceinline__ uint32_t first_node(F3 t0, F3 tm) {
-    uint32_t a = 0;
-    if (t0.x > t0.y) {
-        if (t0.x > t0.z) {
-            if (tm.y < t0.x) a |= 2;
-            if (tm.z < t0.x) a |= 1;
-            return a;
-        }
-    } else {
-        if (t0.y > t0.z) {
-            if (tm.x < t0.y) a |= 4;
-            if (tm.z < t0.y) a |= 1;
-            return a;
-        }
-    }
-    if (tm.x < t0.z) a |= 4;
-    if (tm.y < t0.z) a |= 2;
-    return a;
+    const bool gxy = t0.x > t0.y;
+    const bool ex = gxy && t0.x > t0.z;          // entry plane x
+    const bool ey = !gxy && t0.y > t0.z;         // entry plane y (else z)
+    const float m = ex ? t0.x : (ey ? t0.y : t0.z);
+    return (!ex && tm.x < m ? 4u : 0u) | (!ey && tm.y < m ? 2u : 0u) | ((ex || ey) && tm.z < m ? 1u : 0u);
 }
 
 // Exit plane of child c whose far corner is t1: x -> c|4, y -> c|2, z -> c|1,
 // or 8 when that bit is already set (the successor table at KernelOctree.cpp:165-196,
 // with OctreeNextNode's tie order :62-71).
 __device__ __forceinline__ uint32_t next_node(F3 t1, uint32_t c) {
-    uint32_t nx = (c & 4) ? 8u : (c | 4), ny = (c & 2) ? 8u : (c | 2), nz = (c & 1) ? 8u : (c | 1);
-    if (t1.x < t1.y) {
-        if (t1.x < t1.z) return nx;
-    } else {
-        if (t1.y < t1.z) return ny;
-    }
-    return nz;
+    const uint32_t nx = (c & 4) ? 8u : (c | 4), ny = (c & 2) ? 8u : (c | 2), nz = (c & 1) ? 8u : (c | 1);
+    const bool lxy = t1.x < t1.y, lxz = t1.x < t1.z, lyz = t1.y < t1.z;
+    return lxy ? (lxz ? nx : nz) : (lyz ? ny : nz);
 }
 
 __device__ __forceinline__ void child_interval(F3 t0, F3 t1, uint32_t c, F3 &ct0, F3 &ct1) {

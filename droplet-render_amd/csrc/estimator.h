@@ -165,8 +165,14 @@ __device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Trave
         uint32_t f = stk[T.lvl * stride];
         uint32_t tn = (f >> 24) & 15u;
         if (tn >= 8) {
-            // Pop; recompute the parent's interval from the root along the path.
-            if (--T.lvl < 0) return kTravExhausted;
+            // Pop every exhausted level, then recompute the new top's interval
+            // from the root along the path and visit its next child in this
+            // same step.
+            do {
+                if (--T.lvl < 0) return kTravExhausted;
+                f = stk[T.lvl * stride];
+                tn = (f >> 24) & 15u;
+            } while (tn >= 8);
             F3 a0 = T.rt0, a1 = T.rt1;
 #pragma nounroll   // usually one or two levels; unrolled copies only cost issue slots
             for (int k = 0; k < T.lvl; ++k) {
@@ -176,7 +182,6 @@ __device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Trave
                 a0 = b0; a1 = b1;
             }
             T.t0 = a0; T.t1 = a1;
-            continue;
         }
         uint32_t node = f & kStkNodeMask;
         F3 ct0, ct1;

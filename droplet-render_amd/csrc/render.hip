@@ -647,14 +647,18 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 }
 
 template <int kMode>
-__global__ __launch_bounds__(kBlock) void k_render(RenderParams P, WorkParams W) {
+// Four waves per SIMD (at most 128 VGPRs, a few spilled to scratch): the
+// fourth wave hides more of the traversal's dependent loads than the spills
+// cost (C1: 52.9 -> 49.4 ms).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render(RenderParams P, WorkParams W) {
     __shared__ RenderLds S;
     render_lds_init(S);
     render_body<kMode>(P, W, S);
 }
 
 // The adjoint carries more live state: held to 168 VGPRs so that three
-// waves per SIMD stay resident, as in the forward instantiations.
+// waves per SIMD stay resident (at four it spills too much: the adjoint
+// step of C1 went from 138 to 148 ms).
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render_adjoint(RenderParams P,
                                                                                                     WorkParams W) {
     __shared__ RenderLds S;
@@ -668,7 +672,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // moving on, so the job index is wave-uniform and the job's parameters are
 // read with scalar loads.
 template <int kMode>
-__global__ __launch_bounds__(kBlock) void k_render_batch(const RenderParams *__restrict__ Ps,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_batch(const RenderParams *__restrict__ Ps,
                                                          const WorkParams *__restrict__ Ws, uint32_t njobs) {
     __shared__ RenderLds S;
     render_lds_init(S);

@@ -142,8 +142,10 @@ __device__ __forceinline__ bool octree_init(const drk_octree_node *ob, F3 ro, F3
 // index (x=4,y=2,z=1) -> build slot (x=1,y=2,z=4), a 3-bit reversal.
 __device__ __forceinline__ uint32_t build_slot(uint32_t r) { return ((r & 4) >> 2) | (r & 2) | ((r & 1) << 2); }
 
+// The root interval lives next to the stack in LDS (six floats, `stride`
+// apart), not in registers: it is read only when a pop recomputes an
+// interval from the root.
 struct Traversal {
-    F3 rt0, rt1;      // root interval
     F3 t0, t1;        // interval of the node on top of the stack
     uint32_t a;       // mirror flags
     int lvl;          // top of stack, -1 = exhausted
@@ -157,8 +159,18 @@ enum : int { kTravExhausted = 0, kTravLeaf = 1, kTravPending = 2 };
 // [tra, trb] interval), to its end (kTravExhausted), or by at most `budget`
 // node steps (kTravPending: call again to resume where it stopped).
 // `touches` counts node-record reads (for the algorithmic-bytes model).
-__device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk, uint32_t stride,
-                                         uint32_t &leaf, float &tra, float &trb, uint32_t &touches, uint32_t budget) {
+__device__ __forceinline__ void trav_set_root(float *root, uint32_t stride, F3 t0, F3 t1) {
+    root[0] = t0.x; root[stride] = t0.y; root[2 * stride] = t0.z;
+    root[3 * stride] = t1.x; root[4 * stride] = t1.y; root[5 * stride] = t1.z;
+}
+__device__ __forceinline__ void trav_root(const float *root, uint32_t stride, F3 &t0, F3 &t1) {
+    t0 = mk3(root[0], root[stride], root[2 * stride]);
+    t1 = mk3(root[3 * stride], root[4 * stride], root[5 * stride]);
+}
+
+__device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk,
+                                         const float *root, uint32_t stride, uint32_t &leaf, float &tra, float &trb,
+                                         uint32_t &touches, uint32_t budget) {
     for (uint32_t steps = 0;; ++steps) {
         if (steps >= budget) return kTravPending;
         if (T.lvl < 0) return kTravExhausted;
@@ -173,7 +185,8 @@ __device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Trave
                 f = stk[T.lvl * stride];
                 tn = (f >> 24) & 15u;
             } while (tn >= 8);
-            F3 a0 = T.rt0, a1 = T.rt1;
+            F3 a0, a1;
+            trav_root(root, stride, a0, a1);
 #pragma nounroll   // usually one or two levels; unrolled copies only cost issue slots
             for (int k = 0; k < T.lvl; ++k) {
                 uint32_t ck = (stk[k * stride] >> 28) & 7u;

@@ -127,9 +127,11 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
                                                           uint32_t n, uint32_t max_leaves, uint32_t *nodes,
                                                           float *t0o, float *t1o, uint32_t *count) {
     __shared__ uint32_t stk_all[kMaxDepth * kProbeBlock];
+    __shared__ float root_all[6 * kProbeBlock];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t *stk = stk_all + threadIdx.x;
+    float *root = root_all + threadIdx.x;
     F3 o = mk3(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]);
     F3 d = mk3(rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]);
     Traversal T;
@@ -137,7 +139,8 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
     F3 t0, t1;
     uint32_t a;
     if (octree_init(ob, o, d, t0, t1, a) && !(t1.x < 0.0f || t1.y < 0.0f || t1.z < 0.0f)) {
-        T.rt0 = t0; T.rt1 = t1; T.t0 = t0; T.t1 = t1; T.a = a;
+        trav_set_root(root, kProbeBlock, t0, t1);
+        T.t0 = t0; T.t1 = t1; T.a = a;
         if (is_leaf(ob[0])) {
             if (max_leaves > 0) {
                 nodes[(size_t)i * max_leaves] = 0;
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
             T.lvl = 0;
             uint32_t leaf;
             float a0, a1;
-            while (trav_next(chx, T, stk, kProbeBlock, leaf, a0, a1, touches, ~0u) == kTravLeaf) {
+            while (trav_next(chx, T, stk, root, kProbeBlock, leaf, a0, a1, touches, ~0u) == kTravLeaf) {
                 if (cnt < max_leaves) {
                     nodes[(size_t)i * max_leaves + cnt] = leaf;
                     t0o[(size_t)i * max_leaves + cnt] = a0;

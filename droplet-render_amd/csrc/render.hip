@@ -97,6 +97,10 @@ __device__ __forceinline__ uint32_t fetch_units(uint32_t *counter, bool need) {
 struct RenderLds {
     MieLds mie;
     uint32_t stk[kMaxDepth * kBlock];   // per-lane traversal stacks, lane-interleaved
+    float root[6 * kBlock];             // per-lane root interval of the walk
+    // Per lane, from a scattering event to the end of its shadow ray: the
+    // continuation's direction and throughput and the shadow ray's weight.
+    float cont[9 * kBlock];
     // Adjoint: per wave, the next free slot and the end of the chunk of the
     // event buffer it has reserved (in LDS, so that every lane sees the
     // updates made under divergent control flow).
@@ -121,6 +125,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     unsigned long long (*ev_cur)[2] = S.ev_cur;
 
     uint32_t *stk = S.stk + threadIdx.x;
+    float *root = S.root + threadIdx.x;
+    float *cont = S.cont + threadIdx.x;
     const uint32_t stride = kBlock;
     const KernelConsts &K = *P.kc;
     const float sig_s = P.sigma_s;
@@ -141,7 +147,6 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     F3 beta = mk3(1, 1, 1), pathL = mk3(0, 0, 0), pathE = mk3(0, 0, 0);
     float alpha = 0.0f;
     uint32_t depth_r = 0, kind = RK_PRIMARY, event = 0;
-    F3 cont_dir = mk3(0, 0, 0), cont_beta = mk3(0, 0, 0), shadow_w = mk3(0, 0, 0);
     // Ray.
     F3 ro = mk3(0, 0, 0), rd = mk3(0, 0, 1);
     bool hit = false;
@@ -184,7 +189,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             st = ST_END;   // misses the root: escapes
             return;
         }
-        T.rt0 = t0; T.rt1 = t1; T.t0 = t0; T.t1 = t1; T.a = a;
+        trav_set_root(root, stride, t0, t1);
+        T.t0 = t0; T.t1 = t1; T.a = a;
         if (kCount) cnt_nodes += 1;
         if (is_leaf(P.ob[0])) {
             T.lvl = -2;    // root leaf pending
@@ -374,12 +380,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             bool found, continue_trav = false;
             if (T.lvl == -2) {
                 leaf = 0;
-                tra = fmaxf(fmaxf(T.rt0.x, T.rt0.y), T.rt0.z);
-                trbn = fminf(fminf(T.rt1.x, T.rt1.y), T.rt1.z);
+                F3 r0, r1;
+                trav_root(root, stride, r0, r1);
+                tra = fmaxf(fmaxf(r0.x, r0.y), r0.z);
+                trbn = fminf(fminf(r1.x, r1.y), r1.z);
                 T.lvl = -1;
                 found = true;
             } else {
-                const int r = trav_next(P.chx, T, stk, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
+                const int r = trav_next(P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
                 if (r == kTravPending) continue_trav = true;
                 found = r == kTravLeaf;
             }
@@ -495,14 +503,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             } else if (kind == RK_SHADOW) {
                 if (!hit) {
                     F3 le = lights_eval(K, rd);
-                    pathL.x += shadow_w.x * le.x;
-                    pathL.y += shadow_w.y * le.y;
-                    pathL.z += shadow_w.z * le.z;
+                    pathL.x += cont[6 * stride] * le.x;
+                    pathL.y += cont[7 * stride] * le.y;
+                    pathL.z += cont[8 * stride] * le.z;
                     if ((kAdj && pass == 0) || kRec) vis |= 1ull << event;
                 }
                 // Continue the path along the phase-sampled direction.
-                rd = cont_dir;
-                beta = cont_beta;
+                rd = mk3(cont[0], cont[stride], cont[2 * stride]);
+                beta = mk3(cont[3 * stride], cont[4 * stride], cont[5 * stride]);
                 depth_r += 1;
                 kind = RK_CONT;
                 if (kAdj && pass == 1) wseg = ltot - collected;
@@ -545,9 +553,12 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     F3 w1 = mk3(div_approx(p1.x, p1.x + pd1), div_approx(p1.y, p1.y + pd1), div_approx(p1.z, p1.z + pd1));
                     F3 w2 = mk3(div_approx(p3.x, p3.x + pd2), div_approx(p3.y, p3.y + pd2), div_approx(p3.z, p3.z + pd2));
                     F3 x = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
-                    shadow_w = mk3(beta.x * w2.x * albedo, beta.y * w2.y * albedo, beta.z * w2.z * albedo);
-                    cont_beta = mk3(beta.x * w1.x * albedo, beta.y * w1.y * albedo, beta.z * w1.z * albedo);
-                    cont_dir = srd;
+                    const F3 shadow_w = mk3(beta.x * w2.x * albedo, beta.y * w2.y * albedo, beta.z * w2.z * albedo);
+                    cont[0] = srd.x; cont[stride] = srd.y; cont[2 * stride] = srd.z;
+                    cont[3 * stride] = beta.x * w1.x * albedo;
+                    cont[4 * stride] = beta.y * w1.y * albedo;
+                    cont[5 * stride] = beta.z * w1.z * albedo;
+                    cont[6 * stride] = shadow_w.x; cont[7 * stride] = shadow_w.y; cont[8 * stride] = shadow_w.z;
                     ro = x;
                     rd = lrd;
                     kind = RK_SHADOW;
@@ -647,9 +658,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 }
 
 template <int kMode>
-// Four waves per SIMD (at most 128 VGPRs, a few spilled to scratch): the
-// fourth wave hides more of the traversal's dependent loads than the spills
-// cost (C1: 52.9 -> 49.4 ms).
+// Four waves per SIMD (at most 128 VGPRs; the root interval and the
+// continuation state wait in LDS so that nothing spills): the fourth wave
+// hides more of the traversal's dependent loads (C1: 52.9 -> 46.9 ms).
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render(RenderParams P, WorkParams W) {
     __shared__ RenderLds S;
     render_lds_init(S);
@@ -657,8 +668,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 // The adjoint carries more live state: held to 168 VGPRs so that three
-// waves per SIMD stay resident (at four it spills too much: the adjoint
-// step of C1 went from 138 to 148 ms).
+// waves per SIMD stay resident (squeezed to four it spills, and the C1
+// adjoint step gets slower).
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render_adjoint(RenderParams P,
                                                                                                     WorkParams W) {
     __shared__ RenderLds S;

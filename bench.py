@@ -232,6 +232,10 @@ def main():
             "kernel": "k_render", "kernel_ms": round(kern_avg_ms, 4),
             "alg_bytes_per_launch": int(alg_bytes), "node_reads_per_launch": int(n_node),
             "brick_lookups_per_launch": int(n_look),
+            "hbm_measured_gbs": round(traffic / (kern_avg_ms * 1e6), 1) if traffic else None,
+            "note": "algorithmic bytes of the reference layout (64 B per node record the walk visits, 32 B per "
+                    "trilinear lookup, 32 B per pixel); they are served from L2/MALL, so achieved can exceed the "
+                    "HBM peak; the HBM bytes actually moved are 'traffic' (PMC FETCH_SIZE + WRITE_SIZE)",
         },
         "fwd_adjoint": fwd_adj,
         "cpu_baseline": cpu,

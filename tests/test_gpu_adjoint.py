@@ -197,3 +197,35 @@ def test_event_binning_matches_direct_atomics(pkg, gpu):
         d = np.linalg.norm(out[name] - ref) / np.linalg.norm(ref)
         assert d <= 1e-5, (name, d)
         assert np.array_equal(out[name] == 0, ref == 0), name
+
+
+def test_event_binning_keeps_non_finite_weights(pkg, gpu):
+    """A NaN in the image gradient makes the weights of that pixel's events
+    NaN: the binned path (whose fixed-point sums cannot hold one) must carry
+    them into the same gradient entries as the float atomics do, and leave
+    every other entry as the atomics leave it."""
+    import torch
+    wl = pkg.configs.tiny(n=32, w=32, h=32, spp=8, scattevs=4)
+    wl.scene = raised_scene(pkg, wl)
+    k = wl.kernel()
+    n = wl.w * wl.h * 4
+    rng = np.random.default_rng(4)
+    gl_h = rng.normal(size=n).astype(np.float32)
+    gl_h[(16 * wl.w + 16) * 4] = np.nan   # red channel of the centre pixel
+    gl = torch.tensor(gl_h, device="cuda")
+    ge = torch.tensor(rng.normal(size=n).astype(np.float32), device="cuda")
+    gn = wl.scene.leafx[1] * wl.scene.lvoxc3
+    out = {}
+    for name, budget in [("atomics", 0), ("events", None)]:
+        k.set_event_budget(budget)
+        g = torch.zeros(gn, device="cuda")
+        k.adjoint_device(0, 0, wl.w, wl.h, wl.spp, 21, 0, gl, ge, g)
+        torch.cuda.synchronize()
+        out[name] = g.double().cpu().numpy()
+    k.Destroy()
+    a, e = out["atomics"], out["events"]
+    bad = np.isnan(a)
+    assert bad.any() and not bad.all()
+    assert np.array_equal(np.isnan(e), bad)
+    fin = ~bad
+    assert np.linalg.norm(e[fin] - a[fin]) <= 1e-5 * np.linalg.norm(a[fin])

@@ -283,9 +283,10 @@ void drk_batch_destroy(drk_batch *b);
 /*
  * Extensions with no reference counterpart.
  *
- * drk_kernel_set_tuning: fixes the work-unit chunk (samples per unit) and the
- * persistent grid size instead of the built-in heuristic; 0 keeps the
- * heuristic. Used by the autotuner and benchmarks.
+ * drk_kernel_set_tuning: fixes the work-unit chunk (samples per unit, every
+ * chunk that size; the built-in heuristic instead ends with chunks halving
+ * down to one sample) and the persistent grid size; 0 keeps the heuristic.
+ * Used by the autotuner, benchmarks and tests that need a given chunking.
  *
  * drk_kernel_count: runs the render once with per-lane counters and returns
  * counts[0] = octree node records read, counts[1] = brick lookups. The RNG is

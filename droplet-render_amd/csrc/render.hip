@@ -51,8 +51,9 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 // resumes it in the next round instead of holding up the whole wave), and
 // the ray-end phase, which carries the expensive scattering, sky and camera
 // code, runs only once kEndBatch lanes (at most half the wave's live lanes)
-// are waiting for it or no lane can make progress otherwise. Neither changes what any lane computes, only
-// when; both raise the share of active lanes per instruction.
+// are waiting for it or no lane can make progress otherwise. Neither changes
+// what any lane computes, only when; both raise the share of active lanes
+// per instruction. (The macros exist so that a build can sweep them.)
 #ifndef DRK_TRAV_BUDGET
 #define DRK_TRAV_BUDGET 4
 #endif

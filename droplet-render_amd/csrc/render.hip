@@ -136,8 +136,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
     // Work unit.
     uint32_t st = ST_WORK;
-    uint32_t ray_pend = 0;   // ray id of the pending ray start (ST_RAY)
-    uint32_t chunk_cur = 0, pix_local = 0, px = 0, py = 0, key0 = 0;
+    uint32_t chunk_cur = 0, pix_local = 0, key0 = 0;   // key0: frame pixel index
     const uint32_t band0 = xcc_id();   // first band: this wave's XCD
     uint32_t band_step = 0;            // bands this wave has found drained (wave-uniform)
     uint32_t s_cur = 0, s_end = 0;
@@ -163,8 +162,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     bool first_step = true;
     uint32_t cnt_nodes = 0, cnt_lookups = 0;
     // Adjoint state.
-    F3 gL = mk3(0, 0, 0), gE = mk3(0, 0, 0);
-    float gA = 0.0f, ltot = 0.0f, collected = 0.0f, wseg = 0.0f, seg_a = 0.0f;
+    uint32_t adj_d = 0;   // the pixel's offset in the image-gradient tiles
+    float ltot = 0.0f, collected = 0.0f, wseg = 0.0f, seg_a = 0.0f;
     uint32_t pass = 0;
     uint64_t vis = 0;
     RayRng arng;
@@ -205,7 +204,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
     auto start_path = [&]() {
         bool gm;
-        camera_ray(K, P.w, P.h, px, py, ro, rd, gm);
+        camera_ray(K, P.w, P.h, key0 % P.w, key0 / P.w, ro, rd, gm);
         beta = mk3(1, 1, 1);
         pathL = mk3(0, 0, 0);
         pathE = mk3(0, 0, 0);
@@ -226,18 +225,27 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             rd = sun_sample(K, u01(r4.x), u01(r4.y));
             if (!gm1) { st = ST_END; hit = false; return; }   // untraced lane: unshadowed
             ro = surf;
-            ray_pend = shadow_ray_id(0);
             st = ST_RAY;
             return;
         }
         if (!gm) { st = ST_END; hit = false; return; }
-        ray_pend = 0;
         st = ST_RAY;
     };
 
     // Adjoint: set up the next sample of the current pixel. With a path
     // record, its total and shadow-escape bits come from the forward launch
     // and the replay starts directly at pass 1.
+    // The pixel's image gradient (lights rgb, env rgb; alpha appears in both
+    // buffers), re-read from the tiles where it is needed rather than held
+    // in registers for the whole unit.
+    auto adj_grad = [&](F3 &gL, F3 &gE, float &gA) {
+        const float4 a = reinterpret_cast<const float4 *>(W.adj_l)[adj_d];
+        const float4 c = reinterpret_cast<const float4 *>(W.adj_e)[adj_d];
+        gL = mk3(a.x, a.y, a.z);
+        gE = mk3(c.x, c.y, c.z);
+        gA = a.w + c.w;
+    };
+
     auto adj_begin = [&]() {
         if (W.rec) {
             const size_t n = (size_t)W.npix * P.samples;
@@ -245,6 +253,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             const float4 *R = reinterpret_cast<const float4 *>(W.rec);
             float4 a = R[ri], b = R[n + ri];
             uint2 v = reinterpret_cast<const uint2 *>(R + 2 * n)[ri];
+            F3 gL, gE;
+            float gA;
+            adj_grad(gL, gE, gA);
             ltot = gL.x * a.x + gL.y * a.y + gL.z * a.z + gE.x * b.x + gE.y * b.y + gE.z * b.z + gA * a.w;
             vis = ((uint64_t)v.y << 32) | (uint64_t)v.x;
             pass = 1;
@@ -336,9 +347,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 chunk_cur = u / bp;
                 pix_local = p0 + (u - chunk_cur * bp);
                 uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
-                px = P.x0 + lx;
-                py = P.y0 + ly;
-                key0 = py * P.w + px;
+                key0 = (P.y0 + ly) * P.w + P.x0 + lx;
                 s_cur = chunk_start(W.chunk, W.nfull, P.samples, chunk_cur);
                 s_end = chunk_start(W.chunk, W.nfull, P.samples, chunk_cur + 1);
                 pix_depth = P.depth ? P.depth[key0] : 3.402823466e38f;
@@ -348,12 +357,10 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 sqL[0] = sqL[1] = sqL[2] = 0.0f;
                 sqE[0] = sqE[1] = sqE[2] = 0.0f;
                 if (kAdj) {
-                    size_t d = (size_t)ly * P.tilex + lx;
-                    float4 a = reinterpret_cast<const float4 *>(W.adj_l)[d];
-                    float4 c = reinterpret_cast<const float4 *>(W.adj_e)[d];
-                    gL = mk3(a.x, a.y, a.z);
-                    gE = mk3(c.x, c.y, c.z);
-                    gA = a.w + c.w;   // alpha appears in both buffers
+                    adj_d = ly * P.tilex + lx;
+                    F3 gL, gE;
+                    float gA;
+                    adj_grad(gL, gE, gA);
                     if (gL.x == 0.0f && gL.y == 0.0f && gL.z == 0.0f && gE.x == 0.0f && gE.y == 0.0f &&
                         gE.z == 0.0f && gA == 0.0f)
                         s_cur = s_end;   // no adjoint on this pixel: nothing to differentiate
@@ -372,7 +379,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
         // ---- new camera paths and ray starts, one converged pass each -----
         if (st == ST_PATH) start_path();
-        if (st == ST_RAY) start_ray(ray_pend);
+        if (st == ST_RAY) {
+            // The ray a lane asked for follows from what it is: the camera
+            // ray, the surface shadow ray, or the shadow or continuation ray
+            // of scattering event `event`.
+            start_ray(kind == RK_PRIMARY ? 0u
+                      : kind == RK_SHADOW_PASS ? shadow_ray_id(0)
+                      : kind == RK_SHADOW ? shadow_ray_id(event) : cont_ray_id(event));
+        }
 
         // ---- A: find the next leaf along the ray --------------------------
         if (st == ST_TRAV) {
@@ -515,7 +529,6 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 depth_r += 1;
                 kind = RK_CONT;
                 if (kAdj && pass == 1) wseg = ltot - collected;
-                ray_pend = cont_ray_id(event);
                 st = ST_RAY;
             } else if (!hit) {
                 if (depth_r > 0) {
@@ -528,7 +541,12 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             } else {
                 if (depth_r == 0) {
                     alpha = 1.0f;
-                    if (kAdj && pass == 1) collected += gA;
+                    if (kAdj && pass == 1) {
+                        F3 gL, gE;
+                        float gA;
+                        adj_grad(gL, gE, gA);
+                        collected += gA;
+                    }
                 }
                 if (depth_r >= P.scattevs) {
                     path_done = true;
@@ -568,16 +586,21 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         wseg = 0.0f;
                         if ((vis >> event) & 1ull) {
                             F3 le = lights_eval(K, lrd);
+                            F3 gL, gE;
+                            float gA;
+                            adj_grad(gL, gE, gA);
                             wseg = gL.x * (shadow_w.x * le.x) + gL.y * (shadow_w.y * le.y) + gL.z * (shadow_w.z * le.z);
                             collected += wseg;
                         }
                     }
-                    ray_pend = shadow_ray_id(event);
                     st = ST_RAY;
                 }
             }
             if (path_done && kAdj) {
                 if (pass == 0) {
+                    F3 gL, gE;
+                    float gA;
+                    adj_grad(gL, gE, gA);
                     ltot = gL.x * pathL.x + gL.y * pathL.y + gL.z * pathL.z + gE.x * pathE.x + gE.y * pathE.y +
                            gE.z * pathE.z + gA * alpha;
                     // Replay the same path for the gradients. (A zero total does not
@@ -668,10 +691,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     render_body<kMode>(P, W, S);
 }
 
-// The adjoint carries more live state: held to 168 VGPRs so that three
-// waves per SIMD stay resident (squeezed to four it spills, and the C1
-// adjoint step gets slower).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_render_adjoint(RenderParams P,
+// The adjoint, like the forward, at four waves per SIMD: its image gradient
+// is re-read from the tiles where needed, not kept in registers, so it fits
+// 128 VGPRs without spills (C1 forward+adjoint step 132.8 -> 124.3 ms).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint(RenderParams P,
                                                                                                     WorkParams W) {
     __shared__ RenderLds S;
     render_lds_init(S);

@@ -20,14 +20,36 @@ static __constant__ float c_mie_cdf_rgb[kMieBins * 3] = {
 };
 
 // Tables staged in LDS once per workgroup.
+constexpr uint32_t kMieLutBuckets = 1024;   // u1 buckets of mie_sample's search bounds
+
 struct MieLds {
     float pdf[kMieBins * 3];   // miepdf RGB, KernelSampler.cpp:8-73
     float cdf[kMieBins];       // miecdf R channel, the only one Sample reads (KernelSampler.cpp:262-263)
+    uint16_t lut[kMieLutBuckets];   // per u1 bucket: bounds lo | hi << 8 of mie_sample's search
 };
 
+__device__ __forceinline__ uint32_t mie_search(const MieLds &T, float u1, uint32_t lo, uint32_t hi);
+
+// Stages the tables and the search bounds; every thread of the block calls
+// it (it synchronises the block once inside), and the caller synchronises
+// again before use.
 __device__ __forceinline__ void load_mie(MieLds &m) {
     for (uint32_t i = threadIdx.x; i < kMieBins * 3; i += blockDim.x) m.pdf[i] = c_mie_pdf[i];
     for (uint32_t i = threadIdx.x; i < kMieBins; i += blockDim.x) m.cdf[i] = c_mie_cdf_rgb[i * 3];
+    __syncthreads();
+    // mie_sample's answer grows with u1 except for two steps back by one
+    // (the keys themselves move with u1), so the answers at the smallest and
+    // the largest uniform of a bucket, widened by one each way, bound the
+    // answer for all of it: checked over every uniform the generator can
+    // return (odd multiples of 2^-24). Bucket b holds the uniforms in
+    // [b, b + 1) / kMieLutBuckets.
+    constexpr uint32_t kPer = (1u << 24) / kMieLutBuckets;
+    for (uint32_t b = threadIdx.x; b < kMieLutBuckets; b += blockDim.x) {
+        const float u_lo = (float)(b * kPer + 1u) * 5.9604644775390625e-8f;
+        const float u_hi = (float)((b + 1u) * kPer - 1u) * 5.9604644775390625e-8f;
+        const uint32_t lo = mie_search(m, u_lo, 0u, 255u), hi = mie_search(m, u_hi, 0u, 255u);
+        m.lut[b] = (uint16_t)((lo > 0u ? lo - 1u : 0u) | (min(hi + 1u, 255u) << 8));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -352,15 +374,25 @@ __device__ __forceinline__ float mie_key(const MieLds &T, float ub, uint32_t i, 
 // i = 0..254 and keeps the angle of the last i whose key is below u1; the keys
 // are non-decreasing in i, so a binary search for the first key >= u1 gives
 // the same i.
-__device__ __forceinline__ F3 mie_sample(const MieLds &T, F3 iv, float u1, float u2) {
+// First i in [lo, hi) whose key is >= u1, else hi (keys non-decreasing in i).
+__device__ __forceinline__ uint32_t mie_search(const MieLds &T, float u1, uint32_t lo, uint32_t hi) {
 #pragma clang fp contract(off)
-    float ub = 256.0f * u1;
-    uint32_t lo = 0, hi = 255;
+    const float ub = 256.0f * u1;
     while (lo < hi) {
         uint32_t mid = (lo + hi) >> 1;
         float t;
         if (mie_key(T, ub, mid, t) < u1) lo = mid + 1; else hi = mid;
     }
+    return lo;
+}
+
+__device__ __forceinline__ F3 mie_sample(const MieLds &T, F3 iv, float u1, float u2) {
+#pragma clang fp contract(off)
+    float ub = 256.0f * u1;
+    // The full search over [0, 255] narrowed to the bounds of u1's bucket
+    // (u1 * kMieLutBuckets is exact: a power-of-two scaling).
+    const uint32_t lb = T.lut[(uint32_t)(u1 * (float)kMieLutBuckets)];
+    const uint32_t lo = mie_search(T, u1, lb & 255u, lb >> 8);
     float th = 0.0f;
     if (lo > 0) {
         float t;

@@ -1038,6 +1038,10 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
 bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nbins) {
     shift = 0;
     while (((uint64_t)nbricks + (1ull << shift) - 1) >> shift > kEvMaxBins) ++shift;
+    // Up to 8 bricks per bin while a bin still fits the accumulation LDS:
+    // fewer, fuller bins let the scatter write the sorted copy in longer runs
+    // (C1: 13.9 -> 11.1 ms), at a little more clearing and flushing per chunk.
+    while (shift < 3 && ((uint64_t)lvoxc3 * sizeof(unsigned long long) << (shift + 1)) <= kEvAccumLds) ++shift;
     nbins = (uint32_t)(((uint64_t)nbricks + (1ull << shift) - 1) >> shift);
     return nbricks > 0 && ((uint64_t)lvoxc3 * sizeof(unsigned long long) << shift) <= kEvAccumLds;
 }

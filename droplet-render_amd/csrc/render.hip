@@ -321,12 +321,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     // -W * sigma_e * integral over [a, b] of the trilinear weights, by one
     // uniform point of the adjoint stream (drawn for every covered leaf with a
     // non-zero majorant, so the stream stays aligned with the oracle).
-    auto grad_segment = [&](float a, float b) {
+    // Returns the event's point and weight (0: no event) for grad_at.
+    auto grad_segment = [&](float a, float b, F3 &x) -> float {
         float u = arng.uniform();
-        if (!(b > a)) return;
+        if (!(b > a)) return 0.0f;
         float t = a + u * (b - a);
-        F3 x = mk3(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
-        grad_at(x, -wseg * sig_e * (b - a));
+        x = mk3(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
+        return -wseg * sig_e * (b - a);
     };
 
     for (;;) {
@@ -454,6 +455,11 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         }
 
         // ---- B: one Woodcock step inside the current leaf -----------------
+        // Adjoint pass 1: the step's gradient events (a tracked segment, and a
+        // real collision), emitted after the branches so that the lanes of
+        // every branch share one pass through the event code.
+        F3 ev_x0 = mk3(0, 0, 0), ev_x1 = mk3(0, 0, 0);
+        float ev_w0 = 0.0f, ev_w1 = 0.0f;
         if (st == ST_LEAF) {
             float step = logf(rng.uniform()) / kmaj;
             sr = first_step ? -step : sr - step;
@@ -463,7 +469,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             if (!(sc < tr1)) {
                 td = trb;     // left the leaf: memoryless restart in the next one
                 st = ST_TRAV;
-                if (kAdj && pass == 1 && kmaj > 0.0f) grad_segment(seg_a, trb);
+                if (kAdj && pass == 1 && kmaj > 0.0f) ev_w0 = grad_segment(seg_a, trb, ev_x0);
             } else if (!(sc > tr0)) {
                 hit = true;   // collision before the leaf entry counts as a real one (src/kernel.cpp:287)
                 st = ST_END;
@@ -488,11 +494,16 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     hit = true;
                     st = ST_END;
                     if (kAdj && pass == 1) {
-                        grad_segment(seg_a, td);
-                        grad_at(rc, wseg / p);   // d sigma / sigma at the real collision
+                        ev_w0 = grad_segment(seg_a, td, ev_x0);
+                        ev_x1 = rc;
+                        ev_w1 = wseg / p;   // d sigma / sigma at the real collision
                     }
                 }
             }
+        }
+        if (kAdj) {
+            grad_at(ev_x0, ev_w0);
+            grad_at(ev_x1, ev_w1);
         }
 
         // ---- C: ray finished (escaped or collided) -------------------------

@@ -267,3 +267,29 @@ def test_error_paths_through_abi(pkg, gpu):
         bad = pkg.RenderKernel()
         bad.Initialize(wl.scene, wl.viewi, wl.proji, wl.phase, wl.env, None, 4, -1.0, 0.0, 16, 16, 16, 16, 0,
                        wl.lights)
+
+
+def test_homogeneous_cube_alpha_is_beer_lambert_on_gpu(pkg, oracle, gpu):
+    """The kernel's camera paths scatter with probability 1 - exp(-sigma_t *
+    tau), tau the FP64 optical depth of the trilinear brick density along the
+    ray (the same analytic check as the oracle's, at 8192 paths per pixel)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_functions import optical_depth
+    n = 32
+    g = np.ones((n, n, n), np.float32)
+    spp = 8192
+    wl = pkg.configs.Workload("cube", g, pkg.Scene.from_grid(g), 16, 16, spp, 0, 3.0)
+    out = render_gpu(wl, seed=7)
+    alpha = out[0][..., 3].astype(np.float64) / spp
+    ys, xs = np.mgrid[0:16, 0:16]
+    ro, rd, _ = oracle.camera_rays(wl.viewi, wl.proji, 16, 16, xs.ravel(), ys.ravel())
+    nodes, t0, t1, cnt = oracle.traverse(wl.scene.ob, ro, rd)
+    p = np.array([1 - math.exp(-3.0 * optical_depth(wl.scene, ro[i], rd[i], nodes[i], t0[i], t1[i], cnt[i]))
+                  for i in range(len(ro))])
+    sig = np.sqrt(np.maximum(p * (1 - p), 1e-12) / spp)
+    z = (alpha.ravel() - p) / np.maximum(sig, 1e-9)
+    assert np.abs(z).max() < 5.0, np.abs(z).max()
+    assert abs(z.mean()) < 4.0 / math.sqrt(len(z))
+    assert (cnt > 0).mean() > 0.3

@@ -961,8 +961,8 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
 // w <= 1) is rounded to a multiple of 2^-S with M * 2^S < 2^45, so the at
 // most 2^17 splats of a chunk stay below 2^62, and a splat loses at most
 // 2^-46 M to rounding (float sums lose 2^-24 of the running sum per add).
-// A chunk holding a non-finite weight takes the float path so the NaN or
-// infinity reaches the gradient as before.
+// A chunk holding a non-finite weight takes a float path instead, so that
+// the NaN or infinity reaches the gradient.
 __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     extern __shared__ unsigned long long lq[];
     __shared__ uint32_t s_max, s_bad;

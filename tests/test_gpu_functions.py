@@ -65,8 +65,16 @@ def test_mie_sample_same_bin_as_reference_scan(pkg, oracle, gpu):
     """The kernels' binary search picks the same quantised angle as the
     reference's linear scan for every u."""
     rng = np.random.default_rng(3)
-    n = 50000
-    u1 = ((rng.integers(0, 1 << 23, n) * 2 + 1) * 2.0 ** -24).astype(np.float32)
+    m = rng.integers(0, 1 << 23, 50000) * 2 + 1           # odd multiples of 2^-24
+    # and, for the per-bucket search bounds, every bucket's first and last
+    # uniform and the neighbourhood of the two places where the answer steps
+    # back as u1 grows (u1 ~ 0.98223 and 0.98948)
+    per = (1 << 24) // 1024
+    edges = np.concatenate([np.arange(1024) * per + 1, (np.arange(1024) + 1) * per - 1])
+    dips = np.concatenate([np.arange(-64, 66, 2) + int(round(x * 2 ** 24)) // 2 * 2 + 1 for x in (0.9822312, 0.9894795)])
+    m = np.concatenate([m, edges, dips])
+    n = len(m)
+    u1 = (m * 2.0 ** -24).astype(np.float32)
     u2 = rng.random(n).astype(np.float32)
     iv = np.tile(np.array([[0.48, 0.6, 0.64]], np.float32), (n, 1))
     iv /= np.linalg.norm(iv, axis=1, keepdims=True)

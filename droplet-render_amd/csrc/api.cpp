@@ -47,6 +47,8 @@ struct drk_scene {
     std::vector<drk_octree_node> h_ob;
     drk_octree_node *d_ob = nullptr;
     uint32_t *d_chx = nullptr;      // per node 8 child links: index | kChildLeaf if the child is a leaf
+    float *d_bce = nullptr;         // per FOG brick: its leaf's ce (float4), for the adjoint's event reduction
+    bool bce_ok = true;             // every FOG brick belongs to leaves of one ce (else no event reduction)
     float *d_fog = nullptr;
     float *d_sdf = nullptr;
     uint32_t node_count = 0;
@@ -200,6 +202,24 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
         e = hipMalloc(&s->d_chx, chx.size() * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMemcpy(s->d_chx, chx.data(), chx.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess && leafx_fog) {
+        // Leaf box of every FOG brick: the adjoint's event reduction finds an
+        // event's lookup cell from its point and this box. The reference's
+        // build gives each brick to one leaf; a brick shared by leaves of
+        // different boxes has no single cell, and the adjoint then adds its
+        // gradient with atomics instead of events.
+        std::vector<float> bce((size_t)leafx_fog * 4, 0.0f);
+        std::vector<uint8_t> seen(leafx_fog, 0);
+        for (uint32_t i = 0; i < node_count; ++i) {
+            const uint32_t b = ob[i].volx[DRK_VOLUME_FOG];
+            if (b == DRK_NO_BRICK) continue;
+            if (seen[b] && std::memcmp(&bce[(size_t)b * 4], ob[i].ce, sizeof(ob[i].ce)) != 0) s->bce_ok = false;
+            std::memcpy(&bce[(size_t)b * 4], ob[i].ce, sizeof(ob[i].ce));
+            seen[b] = 1;
+        }
+        e = hipMalloc(&s->d_bce, bce.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(s->d_bce, bce.data(), bce.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
     size_t nb = (size_t)leafx_fog * lvoxc * lvoxc * lvoxc;
     if (e == hipSuccess && nb) e = hipMalloc(&s->d_fog, nb * sizeof(float));
     if (e == hipSuccess && nb) e = hipMemcpy(s->d_fog, pvol_fog, nb * sizeof(float), hipMemcpyHostToDevice);
@@ -234,6 +254,7 @@ void drk_scene_destroy(drk_scene *s) {
     if (!s) return;
     if (s->d_ob) (void)hipFree(s->d_ob);
     if (s->d_chx) (void)hipFree(s->d_chx);
+    if (s->d_bce) (void)hipFree(s->d_bce);
     if (s->d_fog) (void)hipFree(s->d_fog);
     if (s->d_sdf) (void)hipFree(s->d_sdf);
     if (s->d_ev) (void)hipFree(s->d_ev);
@@ -499,13 +520,13 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     EventReduce E;
     std::memset(&E, 0, sizeof(E));
     if (adj) {
-        // Event buffer: 48 bytes per slot (event + sorted copy), sized to 16
+        // Event buffer: 40 bytes per slot (event + sorted copy), sized to 16
         // events per sample of this call within the budget (a multiple of 64
         // slots, which keeps every plane 16-byte aligned); kept for later
         // calls and reallocated when a call wants more.
         uint32_t shift = 0, nbins = 0;
-        const bool binnable = event_bins(k->scene->leafx[1], k->P.lvoxc3, shift, nbins);
         drk_scene *sc = k->scene;
+        const bool binnable = event_bins(sc->leafx[1], k->P.lvoxc3, shift, nbins) && sc->bce_ok;
         size_t budget = sc->ev_budget;
         if (budget == SIZE_MAX) {
             size_t fr = 0, tot = 0;
@@ -536,6 +557,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             E.nbricks = k->scene->leafx[1];
             E.lvoxc = k->P.lvoxc;
             E.lvoxc3 = k->P.lvoxc3;
+            E.bce = reinterpret_cast<const float4 *>(sc->d_bce);
             E.grad = adj->grad;
             HIP_TRY(launch_fill(reinterpret_cast<float *>(sc->d_evbins), 2, 0.0f, s), "adjoint: clear");
         }
@@ -757,7 +779,8 @@ int drk_kernel_footprint(const drk_kernel *k, size_t *bytes) {
 int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
     if (!s || !bytes) return set_error(DRK_EINVAL, "drk_scene_footprint: NULL argument");
     *bytes = (size_t)s->node_count * (sizeof(drk_octree_node) + 8 * sizeof(uint32_t)) +
-             ((size_t)s->leafx[0] + s->leafx[1]) * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
+             ((size_t)s->leafx[0] + s->leafx[1]) * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float) +
+             (s->d_bce ? (size_t)s->leafx[1] * 4 * sizeof(float) : 0);
     if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
     return DRK_OK;
 }

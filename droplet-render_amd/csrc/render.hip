@@ -267,12 +267,11 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
     // Adds weight * (trilinear weight of vertex v at x) to every corner v of
     // the current leaf's gradient brick (adjoint pass 1 only): as one event
-    // {weight, cell} in the event buffer (one 64-bit atomic per wave for the
-    // slots), or, once the buffer is full, as 8 float atomics into the brick.
+    // {weight, x; brick} in the event buffer (one 64-bit atomic per wave for
+    // the slots; the reduction finds the lookup cell), or, once the buffer is
+    // full, as 8 float atomics into the brick.
     auto grad_at = [&](F3 x, float weight) {
         if (weight == 0.0f) return;
-        float nl[3];
-        const uint32_t cell = brick_cell(x, ce, P.lvoxc, nl);
         bool queued = false;
         if (W.ev_cap) {
             // Slots come from the wave's reserved chunk; when it runs out, the
@@ -302,12 +301,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             }
             if (slot < W.ev_cap) {
                 const EventPlanes ev = event_planes(W.ev_base, W.ev_cap);
-                ev.wf[slot] = make_float4(weight, nl[0], nl[1], nl[2]);
-                ev.bc[slot] = make_uint2(leaf_brick, cell);
+                ev.wx[slot] = make_float4(weight, x.x, x.y, x.z);
+                ev.b[slot] = leaf_brick;
                 queued = true;
             }
         }
         if (!queued) {
+            float nl[3];
+            const uint32_t cell = brick_cell(x, ce, P.lvoxc, nl);
             float *g = grad_copy + (size_t)leaf_brick * P.lvoxc3;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
@@ -682,8 +683,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         const unsigned long long end = min(cur[1], (unsigned long long)W.ev_cap);
         const EventPlanes ev = event_planes(W.ev_base, W.ev_cap);
         for (unsigned long long i = cur[0] + lane_id(); i < end; i += 64) {
-            ev.wf[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ev.bc[i] = make_uint2(0u, 0u);
+            ev.wx[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ev.b[i] = 0u;
         }
     }
     if (kCount) {
@@ -880,7 +881,7 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_hist(EventReduce E) {
     const BinTables t = bin_tables(E);
     for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) lh[b] = 0;
     __syncthreads();
-    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.bc[i].x >> E.shift], 1u);
+    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.b[i] >> E.shift], 1u);
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock)
         if (lh[b]) atomicAdd(&t.hist[b], lh[b]);
@@ -936,7 +937,7 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
     const BinTables t = bin_tables(E);
     for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) lh[b] = 0;
     __syncthreads();
-    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.bc[i].x >> E.shift], 1u);
+    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.b[i] >> E.shift], 1u);
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) {
         if (lh[b]) lb[b] = t.bin_start[b] + atomicAdd(&t.bin_fill[b], lh[b]);
@@ -944,11 +945,11 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
     }
     __syncthreads();
     for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) {
-        const uint2 bc = ev.bc[i];
-        const uint32_t b = bc.x >> E.shift;
+        const uint32_t brick = ev.b[i];
+        const uint32_t b = brick >> E.shift;
         const uint32_t pos = lb[b] + atomicAdd(&lh[b], 1u);
-        ev.swf[pos] = ev.wf[i];
-        ev.sbc[pos] = bc;
+        ev.swx[pos] = ev.wx[i];
+        ev.sb[pos] = brick;
     }
 }
 
@@ -988,7 +989,7 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     // non-negative floats) and whether any weight is not finite.
     uint32_t mbits = 0u, bad = 0u;
     for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
-        const float a = fabsf(ev.swf[i].x);
+        const float a = fabsf(ev.swx[i].x);
         bad |= !(a <= 3.402823466e38f);
         mbits = max(mbits, __float_as_uint(a));
     }
@@ -1003,16 +1004,17 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
         for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lg[j] = 0.0f;
         __syncthreads();
         for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
-            const float4 wf = ev.swf[i];
-            const uint2 bc = ev.sbc[i];
-            const float nl[3] = {wf.y, wf.z, wf.w};
-            float *g = lg + (bc.x - brick0) * E.lvoxc3;
+            const float4 wx = ev.swx[i];
+            const uint32_t brick = ev.sb[i];
+            float nl[3];
+            const uint32_t cell = brick_cell(mk3(wx.y, wx.z, wx.w), E.bce[brick], E.lvoxc, nl);
+            float *g = lg + (brick - brick0) * E.lvoxc3;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 uint32_t idx;
                 float w;
-                cell_corner(bc.y, nl, E.lvoxc, v, idx, w);
-                atomicAdd(g + idx, wf.x * w);
+                cell_corner(cell, nl, E.lvoxc, v, idx, w);
+                atomicAdd(g + idx, wx.x * w);
             }
         }
         __syncthreads();
@@ -1024,18 +1026,19 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     }
     const int sh = 44 - ilogbf(__uint_as_float(s_max));   // M * 2^sh < 2^45
     for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
-        const float4 wf = ev.swf[i];
-        const uint2 bc = ev.sbc[i];
-        const float nl[3] = {wf.y, wf.z, wf.w};
-        unsigned long long *g = lq + (bc.x - brick0) * E.lvoxc3;
+        const float4 wx = ev.swx[i];
+        const uint32_t brick = ev.sb[i];
+        float nl[3];
+        const uint32_t cell = brick_cell(mk3(wx.y, wx.z, wx.w), E.bce[brick], E.lvoxc, nl);
+        unsigned long long *g = lq + (brick - brick0) * E.lvoxc3;
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
             uint32_t idx;
             float w;
-            cell_corner(bc.y, nl, E.lvoxc, v, idx, w);
+            cell_corner(cell, nl, E.lvoxc, v, idx, w);
             // |splat| * 2^sh < 2^45: scaled exactly, rounded to an integer
             // exactly representable in float, converted exactly.
-            const long long q = (long long)rintf(ldexpf(wf.x * w, sh));
+            const long long q = (long long)rintf(ldexpf(wx.x * w, sh));
             atomicAdd(g + idx, (unsigned long long)q);
         }
     }

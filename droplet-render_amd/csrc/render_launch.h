@@ -110,27 +110,31 @@ struct EventReduce {
     uint32_t ev_cap;
     uint32_t nbins, shift; // bin = brick >> shift
     uint32_t nbricks, lvoxc, lvoxc3;
+    const float4 *bce;     // per FOG brick: centre and half-size of its leaf
     float *grad;           // accumulated into
 };
 
-// Planes of the event buffer (cap slots each): events {weight, fractions}
-// float4 and {brick, cell} uint2, then their copies sorted by bin.
+// Planes of the event buffer (cap slots each): events {weight, point x, y, z}
+// float4 and their FOG brick, then their copies sorted by bin. The lookup
+// cell and lerp fractions are not stored: the accumulation recomputes them
+// from the point and the brick's leaf box (EventReduce::bce) with the same
+// arithmetic as the lookup.
 struct EventPlanes {
-    float4 *wf;
-    uint2 *bc;
-    float4 *swf;
-    uint2 *sbc;
+    float4 *wx;
+    uint32_t *b;
+    float4 *swx;
+    uint32_t *sb;
 };
 __host__ __device__ inline EventPlanes event_planes(char *base, uint32_t cap) {
     EventPlanes e;
-    e.wf = reinterpret_cast<float4 *>(base);
-    e.bc = reinterpret_cast<uint2 *>(base + (size_t)cap * 16);
-    e.swf = reinterpret_cast<float4 *>(base + (size_t)cap * 24);
-    e.sbc = reinterpret_cast<uint2 *>(base + (size_t)cap * 40);
+    e.wx = reinterpret_cast<float4 *>(base);
+    e.b = reinterpret_cast<uint32_t *>(base + (size_t)cap * 16);
+    e.swx = reinterpret_cast<float4 *>(base + (size_t)cap * 20);
+    e.sb = reinterpret_cast<uint32_t *>(base + (size_t)cap * 36);
     return e;
 }
 constexpr uint32_t kEvMaxBins = 8192;          // LDS histogram of the sort
-constexpr size_t kEventSlotBytes = 48;         // event (24 B) + its sorted copy (24 B)
+constexpr size_t kEventSlotBytes = 40;         // event (20 B) + its sorted copy (20 B)
 constexpr uint32_t kEvChunk = 2048;            // event slots a wave reserves at a time
 constexpr uint32_t kEvAccumLds = 65536;        // bytes of LDS gradient per accumulation block
 // Largest shift (bricks per bin = 2^shift) and bin count for a scene, or

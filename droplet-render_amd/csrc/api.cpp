@@ -43,6 +43,11 @@ static int hip_fail(hipError_t e, const char *what) {
 
 using namespace drk;
 
+// Default work-unit chunking of a launch: at most this many sample chunks per
+// pixel, and at most this many bytes of per-chunk partial sums.
+constexpr uint64_t kMaxAutoChunks = 64;
+constexpr uint64_t kMaxPartialBytes = 4ull << 30;
+
 struct drk_scene {
     std::vector<drk_octree_node> h_ob;
     drk_octree_node *d_ob = nullptr;
@@ -485,19 +490,26 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     const uint32_t npix = rx * ry;
     const uint32_t bpc = mode == kModeForwardSq ? k->blocks_per_cu : (uint32_t)render_blocks_per_cu(mode);
     uint32_t grid = k->grid_override ? k->grid_override : k->cu_count * std::max(1u, bpc);
-    const uint64_t lanes = (uint64_t)grid * render_block_size();
     uint32_t chunk = k->chunk_override;
     if (chunk == 0) {
-        // Aim for >= 16 units per lane so the tail of the drain stays short.
-        uint64_t want_units = lanes * 16;
-        uint64_t nch = (want_units + npix - 1) / npix;
-        nch = std::max<uint64_t>(1, std::min<uint64_t>(nch, samples));
+        // Short units keep the drain at the end of a launch short (the last
+        // units handed out are single samples of a pixel rather than
+        // several samples of an expensive one), and the waves' unit reserves
+        // make a refill cheap. Up to kMaxAutoChunks chunks (one sample each
+        // for up to that many samples per pixel), fewer where the per-chunk
+        // partial sums would exceed kMaxPartialBytes.
+        const uint64_t bytes_per_chunk = (uint64_t)npix * 16 * (sumsq ? 4 : 2);
+        uint64_t nch = std::min<uint64_t>(samples, kMaxAutoChunks);
+        nch = std::min<uint64_t>(nch, std::max<uint64_t>(1, kMaxPartialBytes / std::max<uint64_t>(bytes_per_chunk, 1)));
+        nch = std::max<uint64_t>(1, nch);
         chunk = (uint32_t)((samples + nch - 1) / nch);
     }
     chunk = std::max(1u, std::min(chunk, samples));
     uint32_t nfull, nchunks;
     plan_chunks(samples, chunk, k->chunk_override == 0, nfull, nchunks);
-    if ((uint64_t)npix * nchunks > 0xffffffffull) return set_error(DRK_EINVAL, "render: too many work units");
+    // Unit indices stay below 2^32 with room for the fetches that run past a
+    // band's end (at most a wave's lanes plus its reserve per wave and band).
+    if ((uint64_t)npix * nchunks > 0xf0000000ull) return set_error(DRK_EINVAL, "render: too many work units");
     WorkParams W;
     std::memset(&W, 0, sizeof(W));
     W.counter = k->d_counter;
@@ -907,7 +919,7 @@ static int batch_plan(drk_batch *b) {
         }
         uint32_t nfull, nchunks;
         plan_chunks(J.samples, chunk, k->chunk_override == 0, nfull, nchunks);
-        if ((uint64_t)npix * nchunks > 0xffffffffull) return set_error(DRK_EINVAL, "batch: job %u has too many work units", j);
+        if ((uint64_t)npix * nchunks > 0xf0000000ull) return set_error(DRK_EINVAL, "batch: job %u has too many work units", j);
         W.counter = b->d_counters + (size_t)j * kMaxBands * kCounterStride;
         W.nbands = k->bands;
         W.npix = npix;

@@ -57,6 +57,9 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 #ifndef DRK_TRAV_BUDGET
 #define DRK_TRAV_BUDGET 4
 #endif
+#ifndef DRK_UNIT_RESERVE
+#define DRK_UNIT_RESERVE 32
+#endif
 #ifndef DRK_END_BATCH
 #define DRK_END_BATCH 32
 #endif
@@ -82,16 +85,40 @@ __device__ __forceinline__ uint32_t cu_slot() {
     return cu | (se << 4);
 }
 
-// Wave-aggregated fetch of work units for the lanes that need one.
-__device__ __forceinline__ uint32_t fetch_units(uint32_t *counter, bool need) {
-    uint64_t mask = __ballot(need);
-    uint32_t lane = lane_id();
-    uint32_t base = 0;
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-    return base + below;
+// Work units for the lanes that need one, from the wave's reserve first; a
+// fetch (one atomic for the wave) takes what the reserve lacks plus r_extra
+// more units for later refills, so that most refills do not wait for a
+// device-scope atomic. The reserve is kept while the band hands out full
+// chunks (chunk index < nfull) and not in the tail of short chunks, where
+// units held by one wave would lengthen the drain at the end of a launch.
+// Units are indices into the band's [0, total); a unit >= total means the
+// band is drained. r_base, r_cnt and r_extra are wave-uniform.
+constexpr uint32_t kUnitReserve = DRK_UNIT_RESERVE;
+__device__ __forceinline__ uint32_t take_units(uint32_t *counter, bool need, uint32_t bp, uint32_t nfull,
+                                               uint32_t total, uint32_t &r_base, uint32_t &r_cnt,
+                                               uint32_t &r_extra) {
+    const uint64_t mask = __ballot(need);
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    uint32_t u;
+    if (cnt <= r_cnt) {
+        u = r_base + rank;
+        r_base += cnt;
+        r_cnt -= cnt;
+    } else {
+        const uint32_t req = cnt - r_cnt + r_extra;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t fresh = 0;
+        if (lane_id() == leader) fresh = atomicAdd(counter, req);
+        fresh = __builtin_amdgcn_readfirstlane(__shfl(fresh, (int)leader));
+        u = rank < r_cnt ? r_base + rank : fresh + (rank - r_cnt);
+        r_base = fresh + (cnt - r_cnt);
+        r_cnt = r_extra;
+        r_extra = fresh < nfull * bp ? kUnitReserve : 0u;   // still in the full chunks
+    }
+    // Units past the band's end are not held.
+    r_cnt = r_base < total ? min(r_cnt, total - r_base) : 0u;
+    return u;
 }
 
 // Workgroup-shared state of the path tracer.
@@ -139,6 +166,10 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     uint32_t chunk_cur = 0, pix_local = 0, key0 = 0;   // key0: frame pixel index
     const uint32_t band0 = xcc_id();   // first band: this wave's XCD
     uint32_t band_step = 0;            // bands this wave has found drained (wave-uniform)
+    // The wave's reserve of fetched units of the current band, [r_base,
+    // r_base + r_cnt), and how many extra units its next fetch asks for
+    // (wave-uniform).
+    uint32_t r_base = 0, r_cnt = 0, r_extra = 0;
     uint32_t s_cur = 0, s_end = 0;
     float pix_depth = 3.402823466e38f;
     float accL[3] = {0, 0, 0}, accE[3] = {0, 0, 0}, accA = 0.0f;
@@ -344,8 +375,10 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             const uint32_t b = (band0 + band_step) % W.nbands;
             const uint32_t p0 = (uint32_t)((uint64_t)b * W.npix / W.nbands);
             const uint32_t bp = (uint32_t)((uint64_t)(b + 1) * W.npix / W.nbands) - p0;
-            const uint32_t u = fetch_units(W.counter + kCounterStride * b, need);
-            if (need && u < bp * W.nchunks) {
+            const uint32_t total = bp * W.nchunks;
+            const uint32_t u = take_units(W.counter + kCounterStride * b, need, bp, W.nfull, total, r_base, r_cnt,
+                                          r_extra);
+            if (need && u < total) {
                 chunk_cur = u / bp;
                 pix_local = p0 + (u - chunk_cur * bp);
                 uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
@@ -375,7 +408,11 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 }
             }
             // Lanes that drew past the band's end leave it for the next one.
-            if (__ballot(need && !(u < bp * W.nchunks))) ++band_step;
+            if (__ballot(need && !(u < total))) {
+                ++band_step;
+                r_cnt = 0;
+                r_extra = 0;
+            }
         }
         if (!__ballot(st != ST_DONE)) break;
 

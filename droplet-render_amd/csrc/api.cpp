@@ -47,6 +47,8 @@ using namespace drk;
 // pixel, and at most this many bytes of per-chunk partial sums.
 constexpr uint64_t kMaxAutoChunks = 64;
 constexpr uint64_t kMaxPartialBytes = 4ull << 30;
+// Batched launches: sample chunks per pixel of every job at unit scale 1.
+constexpr double kBatchChunks = 16.0;
 
 struct drk_scene {
     std::vector<drk_octree_node> h_ob;
@@ -106,7 +108,6 @@ struct drk_batch {
     std::vector<drk_batch_job> jobs;
     std::vector<RenderParams> hP;
     std::vector<WorkParams> hW;
-    std::vector<uint64_t> work;      // pixel samples per job
     RenderParams *dP = nullptr;
     WorkParams *dW = nullptr;
     uint32_t *d_counters = nullptr;  // kMaxBands * kCounterStride words per job
@@ -114,7 +115,7 @@ struct drk_batch {
     size_t partial_bytes = 0;
     uint32_t grid = 0;               // 0: occupancy-derived
     uint32_t grid_eff = 1;           // grid of the planned launch
-    float unit_scale = 1.0f;         // work units per lane, relative to the default of 16
+    float unit_scale = 1.0f;         // sample chunks per pixel, relative to kBatchChunks
     uint32_t cu_count = 1;
     bool sumsq = false;
     bool planned = false;
@@ -878,10 +879,6 @@ static int batch_plan(drk_batch *b) {
     const int mode = b->sumsq ? kModeForwardSq : kModeForward;
     const uint32_t grid = b->grid ? b->grid : b->cu_count * (uint32_t)std::max(1, render_batch_blocks_per_cu(mode));
     b->grid_eff = grid;
-    const uint64_t lanes = (uint64_t)grid * render_block_size();
-    uint64_t total = 0;
-    for (uint64_t w : b->work) total += w;
-    const double want = std::max(1.0, (double)lanes * 16.0 * b->unit_scale);
     size_t poff = 0;
     std::vector<size_t> pbase(nj, 0);
     for (uint32_t j = 0; j < nj; ++j) {
@@ -909,9 +906,13 @@ static int batch_plan(drk_batch *b) {
             if (k->chunk_override) {
                 chunk = k->chunk_override;
             } else {
-                // Work units in proportion to the job's share of the samples.
-                const double units = want * (double)b->work[j] / (double)std::max<uint64_t>(total, 1);
-                uint64_t nch = (uint64_t)std::ceil(units / npix);
+                // kBatchChunks * unit_scale sample chunks per pixel for every
+                // job (units of equal sample counts, short enough that the
+                // drain at the end of the launch stays short), fewer where
+                // the job's partial sums would exceed kMaxPartialBytes.
+                const uint64_t bytes_per_chunk = (uint64_t)npix * 16 * (b->sumsq ? 4 : 2);
+                uint64_t nch = (uint64_t)std::llround(kBatchChunks * b->unit_scale);
+                nch = std::min<uint64_t>(nch, kMaxPartialBytes / std::max<uint64_t>(bytes_per_chunk, 1));
                 nch = std::max<uint64_t>(1, std::min<uint64_t>(nch, J.samples));
                 chunk = (uint32_t)((J.samples + nch - 1) / nch);
             }
@@ -1006,11 +1007,6 @@ int drk_batch_create(const drk_batch_job *jobs, uint32_t njobs, drk_batch **out)
     b->hP.resize(njobs);
     b->hW.resize(njobs);
     b->sumsq = sumsq;
-    for (uint32_t j = 0; j < njobs; ++j) {
-        uint32_t rx, ry;
-        rendered_extent(jobs[j].kernel, jobs[j].x0, jobs[j].y0, jobs[j].tilex, jobs[j].tiley, rx, ry);
-        b->work.push_back((uint64_t)rx * ry * jobs[j].samples);
-    }
     b->cu_count = jobs[0].kernel->cu_count;
     hipError_t e = hipMalloc(&b->dP, sizeof(RenderParams) * njobs);
     if (e == hipSuccess) e = hipMalloc(&b->dW, sizeof(WorkParams) * njobs);

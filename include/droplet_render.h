@@ -259,9 +259,10 @@ int drk_kernel_adjoint_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, ui
  * the same launch sequence into a hipGraph on its first call and replays
  * the graph after that (a graph made before drk_batch_set_tuning or
  * drk_batch_autotune is rebuilt). drk_batch_set_tuning sets the grid
- * (workgroups; 0 = occupancy) and the work units per lane relative to the
- * default; drk_batch_autotune times a grid x units sweep on the stream
- * (synchronising it), keeps the fastest and writes the table to report.
+ * (workgroups; 0 = occupancy) and the work-unit granularity: unit_scale x 16
+ * sample chunks per pixel of every job (default 1); drk_batch_autotune
+ * times a grid x units sweep on the stream (synchronising it), keeps the
+ * fastest and writes the table to report.
  * Per-job results equal those of separate launches with the same chunking.
  */
 typedef struct drk_batch drk_batch;

@@ -99,6 +99,32 @@ def test_determinism_and_tiling(pkg, gpu):
     k.Destroy()
 
 
+def test_every_work_unit_runs_once(pkg, gpu):
+    """Work units come from per-wave reserves refilled by one atomic per
+    wave; whatever the chunking and the grid, every (pixel, sample) path runs
+    exactly once. The counting pass adds up each path's node reads and brick
+    lookups, so a dropped or repeated unit changes the totals; images with the
+    same chunking are bit-identical across grids."""
+    wl = pkg.configs.tiny(w=64, h=48, spp=12)
+    k = wl.kernel()
+    ref = None
+    for chunk, grid in [(0, 0), (12, 0), (1, 0), (1, 2), (5, 3), (0, 1), (2, 7)]:
+        k.set_tuning(chunk=chunk, grid_blocks=grid)
+        c = k.count(0, 0, wl.w, wl.h, wl.spp, 9)
+        assert c[0] > 0 and c[1] > 0
+        if ref is None:
+            ref = c
+        assert c == ref, (chunk, grid, c, ref)
+    imgs = []
+    for grid in [0, 2, 5]:
+        k.set_tuning(chunk=1, grid_blocks=grid)
+        imgs.append(render_gpu(wl, 9, k=k))
+    for other in imgs[1:]:
+        for a, b in zip(imgs[0], other):
+            assert np.array_equal(a, b)
+    k.Destroy()
+
+
 def test_progressive_sample_base(pkg, gpu):
     """Two Render calls of spp/2 (the second continuing the sample counter)
     equal one call of spp up to summation order, like the add-on's progressive

@@ -963,30 +963,107 @@ __global__ __launch_bounds__(1024) void k_ev_scan(EventReduce E) {
     }
 }
 
+// Exclusive prefix sum of a[0, n) in place by the whole block (kEvSortBlock
+// threads; wsum: 16 words of LDS). Ends with a barrier.
+__device__ __forceinline__ void block_exscan(uint32_t *a, uint32_t n, uint32_t *wsum) {
+    const uint32_t per = (n + kEvSortBlock - 1) / kEvSortBlock;
+    const uint32_t b0 = min(threadIdx.x * per, n), b1 = min(b0 + per, n);
+    uint32_t s = 0;
+    for (uint32_t b = b0; b < b1; ++b) s += a[b];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = s;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t base = x - s;
+    for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t v = a[b];
+        a[b] = base;
+        base += v;
+    }
+    __syncthreads();
+}
+
+// Writes every event of the block's kEvSortChunk to its slot in the copy
+// sorted by bin. Each bin's run of the block is reserved with one atomic;
+// the events then go through LDS kEvTile at a time, ordered by bin (a
+// counting sort of the tile), so that consecutive lanes store consecutive
+// slots of a bin's run. Stored straight from their lanes (one cache line per
+// lane), the 16-byte events kept the texture addresser busy 80% of the
+// kernel's time (C1: 9.4 ms; through the tiles 7.2 ms; tiles of 3072 or 6144
+// events measured no faster). LDS: two words per bin (<= 8192 bins) and
+// 20 B per tile slot, at most 104 KB.
+constexpr uint32_t kEvTile = 2048;
+__host__ __device__ constexpr size_t ev_scatter_lds(uint32_t nbins) {
+    return ((size_t)2 * nbins + 16 + 3) / 4 * 16 + (size_t)kEvTile * (16 + 4);
+}
 __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
     extern __shared__ uint32_t lds[];
-    uint32_t *lh = lds, *lb = lds + E.nbins;
+    const uint32_t nb = E.nbins;
+    uint32_t *cur = lds, *th = lds + nb, *wsum = lds + 2 * nb;
+    float4 *tw = reinterpret_cast<float4 *>(lds + (2 * nb + 16 + 3) / 4 * 4);
+    uint32_t *tb = reinterpret_cast<uint32_t *>(tw + kEvTile);
     const uint32_t n = ev_n(E);
     const uint64_t start = (uint64_t)blockIdx.x * kEvSortChunk;
     if (start >= n) return;
     const uint32_t end = (uint32_t)min<uint64_t>(start + kEvSortChunk, n);
     const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
     const BinTables t = bin_tables(E);
-    for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) lh[b] = 0;
+    for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock) th[b] = 0;
     __syncthreads();
-    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&lh[ev.b[i] >> E.shift], 1u);
+    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&th[ev.b[i] >> E.shift], 1u);
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < E.nbins; b += kEvSortBlock) {
-        if (lh[b]) lb[b] = t.bin_start[b] + atomicAdd(&t.bin_fill[b], lh[b]);
-        lh[b] = 0;
-    }
-    __syncthreads();
-    for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) {
-        const uint32_t brick = ev.b[i];
-        const uint32_t b = brick >> E.shift;
-        const uint32_t pos = lb[b] + atomicAdd(&lh[b], 1u);
-        ev.swx[pos] = ev.wx[i];
-        ev.sb[pos] = brick;
+    for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock)
+        cur[b] = th[b] ? t.bin_start[b] + atomicAdd(&t.bin_fill[b], th[b]) : 0u;
+    constexpr uint32_t kPer = kEvTile / kEvSortBlock;
+    for (uint32_t t0 = (uint32_t)start; t0 < end; t0 += kEvTile) {
+        const uint32_t tn = min(kEvTile, end - t0);
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock) th[b] = 0;
+        __syncthreads();
+        uint32_t br[kPer], rk[kPer];
+        float4 w[kPer];
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kEvSortBlock;
+            if (j < tn) {
+                br[u] = ev.b[t0 + j];
+                w[u] = ev.wx[t0 + j];
+                rk[u] = atomicAdd(&th[br[u] >> E.shift], 1u);
+            }
+        }
+        __syncthreads();
+        block_exscan(th, nb, wsum);   // th: the tile's first slot of every bin
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            const uint32_t j = threadIdx.x + u * kEvSortBlock;
+            if (j < tn) {
+                const uint32_t p = th[br[u] >> E.shift] + rk[u];
+                tb[p] = br[u];
+                tw[p] = w[u];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            const uint32_t p = threadIdx.x + u * kEvSortBlock;
+            if (p < tn) {
+                const uint32_t brick = tb[p];
+                const uint32_t b = brick >> E.shift;
+                const uint32_t g = cur[b] + (p - th[b]);
+                ev.swx[g] = tw[p];
+                ev.sb[g] = brick;
+            }
+        }
+        __syncthreads();
+        // Advance the block's cursors by the tile's counts.
+        for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock)
+            cur[b] += (b + 1 < nb ? th[b + 1] : tn) - th[b];
     }
 }
 
@@ -1104,7 +1181,7 @@ hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
     k_fill_u32<<<(E.nbins + 255) / 256, 256, 0, s>>>(E.bins + 2, E.nbins, 0u);
     k_ev_hist<<<sort_blocks, kEvSortBlock, E.nbins * sizeof(uint32_t), s>>>(E);
     k_ev_scan<<<1, 1024, 0, s>>>(E);
-    k_ev_scatter<<<sort_blocks, kEvSortBlock, 2 * E.nbins * sizeof(uint32_t), s>>>(E);
+    k_ev_scatter<<<sort_blocks, kEvSortBlock, ev_scatter_lds(E.nbins), s>>>(E);
     const size_t lds = (size_t)E.lvoxc3 * sizeof(unsigned long long) << E.shift;
     k_ev_accum<<<acc_blocks, kEvAccBlock, lds, s>>>(E);
     return hipGetLastError();

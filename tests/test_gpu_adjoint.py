@@ -199,6 +199,42 @@ def test_event_binning_matches_direct_atomics(pkg, gpu):
         assert np.array_equal(out[name] == 0, ref == 0), name
 
 
+def test_event_binning_with_thousands_of_bins(pkg, gpu):
+    """A scene of 32768 bricks (every leaf of a 264^3 lattice occupied) sorts
+    its events into 4096 bins: the scatter's bin scan spans several bins per
+    thread and its LDS passes 64 KB. Binned events still give the gradient of
+    per-corner float atomics."""
+    import torch
+    g = np.zeros((264, 264, 264), np.float32)
+    g[::3, ::3, ::3] = 0.5
+    sc = pkg.scene.Scene.from_grid(g)
+    assert sc.leafx[1] == 32768
+    wl = pkg.configs.Workload("lattice264", g, sc, 48, 40, 8, 4, 8.0)
+    k = wl.kernel()
+    n = wl.w * wl.h * 4
+    rng = np.random.default_rng(4)
+    gl = torch.tensor(rng.normal(size=n).astype(np.float32) * 100.0, device="cuda")
+    ge = torch.tensor(rng.normal(size=n).astype(np.float32), device="cuda")
+    gn = sc.leafx[1] * sc.lvoxc3
+    out = {}
+    for name, budget in [("atomics", 0), ("events", None)]:
+        k.set_event_budget(budget)
+        gr = torch.zeros(gn, device="cuda")
+        k.adjoint_device(0, 0, wl.w, wl.h, wl.spp, 21, 0, gl, ge, gr)
+        torch.cuda.synchronize()
+        out[name] = gr.double().cpu().numpy()
+    k.Destroy()
+    ref = out["atomics"]
+    assert np.isfinite(ref).all() and np.count_nonzero(ref) > 10000
+    d = np.linalg.norm(out["events"] - ref) / np.linalg.norm(ref)
+    assert d <= 1e-5, d
+    # Zeros agree except for entries whose splats are below the fixed-point
+    # quantum of their chunk (2^-44 of the chunk's largest weight): a corner
+    # weight (1 - nl)(1 - nl') of ~1e-14 rounds to zero there.
+    tiny = np.abs(ref) <= 2.0 ** -40 * np.abs(ref).max()
+    assert np.all(((out["events"] == 0) == (ref == 0)) | tiny)
+
+
 def test_event_binning_keeps_non_finite_weights(pkg, gpu):
     """A NaN in the image gradient makes the weights of that pixel's events
     NaN: the binned path (whose fixed-point sums cannot hold one) must carry

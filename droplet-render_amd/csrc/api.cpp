@@ -45,7 +45,10 @@ using namespace drk;
 
 // Default work-unit chunking of a launch: at most this many sample chunks per
 // pixel, and at most this many bytes of per-chunk partial sums.
-constexpr uint64_t kMaxAutoChunks = 64;
+#ifndef DRK_MAX_AUTO_CHUNKS
+#define DRK_MAX_AUTO_CHUNKS 64
+#endif
+constexpr uint64_t kMaxAutoChunks = DRK_MAX_AUTO_CHUNKS;
 constexpr uint64_t kMaxPartialBytes = 4ull << 30;
 // Batched launches: sample chunks per pixel of every job at unit scale 1.
 constexpr double kBatchChunks = 16.0;

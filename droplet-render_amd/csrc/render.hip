@@ -1150,9 +1150,12 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
             uint32_t idx;
             float w;
             cell_corner(cell, nl, E.lvoxc, v, idx, w);
-            // |splat| * 2^sh < 2^45: scaled exactly, rounded to an integer
-            // exactly representable in float, converted exactly.
-            const long long q = (long long)rintf(ldexpf(wx.x * w, sh));
+            // |splat| * 2^sh < 2^45: scaled exactly in double and rounded to
+            // the nearest integer (ties to even) by adding 1.5 * 2^52, whose
+            // ulp is 1; the integer is then the low bits of the sum. Equal to
+            // (long long)rintf(ldexpf(splat, sh)) in fewer instructions.
+            const double r = ldexp((double)(wx.x * w), sh) + 6755399441055744.0;
+            const long long q = __double_as_longlong(r) - 0x4338000000000000ll;
             atomicAdd(g + idx, (unsigned long long)q);
         }
     }

@@ -544,6 +544,31 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             grad_at(ev_x1, ev_w1);
         }
 
+        // ---- C0: shadow ray finished: its contribution, then the path's
+        // continuation. In the adjoint this cheap step runs every round
+        // rather than waiting for the ray-end batch: its replay ends the
+        // shadow rays that were blocked in the forward pass at once, and
+        // those lanes would otherwise idle until a batch formed (C1
+        // fwd+adjoint 100.5 -> 96.8 ms; the forward, whose shadow rays all
+        // trace, is 1.2 ms slower this way and keeps them in the batch). -----
+        auto shadow_end = [&]() {
+            if (!hit) {
+                F3 le = lights_eval(K, rd);
+                pathL.x += cont[6 * stride] * le.x;
+                pathL.y += cont[7 * stride] * le.y;
+                pathL.z += cont[8 * stride] * le.z;
+                if ((kAdj && pass == 0) || kRec) vis |= 1ull << event;
+            }
+            // Continue the path along the phase-sampled direction.
+            rd = mk3(cont[0], cont[stride], cont[2 * stride]);
+            beta = mk3(cont[3 * stride], cont[4 * stride], cont[5 * stride]);
+            depth_r += 1;
+            kind = RK_CONT;
+            if (kAdj && pass == 1) wseg = ltot - collected;
+            st = ST_RAY;
+        };
+        if (kAdj && st == ST_END && kind == RK_SHADOW) shadow_end();
+
         // ---- C: ray finished (escaped or collided) -------------------------
         // (The batch shrinks with the wave's remaining lanes, so that in the
         // drain at the end of a launch the last paths do not wait for each
@@ -564,21 +589,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 }
                 alpha = 1.0f;
                 path_done = true;
-            } else if (kind == RK_SHADOW) {
-                if (!hit) {
-                    F3 le = lights_eval(K, rd);
-                    pathL.x += cont[6 * stride] * le.x;
-                    pathL.y += cont[7 * stride] * le.y;
-                    pathL.z += cont[8 * stride] * le.z;
-                    if ((kAdj && pass == 0) || kRec) vis |= 1ull << event;
-                }
-                // Continue the path along the phase-sampled direction.
-                rd = mk3(cont[0], cont[stride], cont[2 * stride]);
-                beta = mk3(cont[3 * stride], cont[4 * stride], cont[5 * stride]);
-                depth_r += 1;
-                kind = RK_CONT;
-                if (kAdj && pass == 1) wseg = ltot - collected;
-                st = ST_RAY;
+            } else if (!kAdj && kind == RK_SHADOW) {
+                shadow_end();
             } else if (!hit) {
                 if (depth_r > 0) {
                     F3 lc = lights_eval(K, rd);

@@ -655,6 +655,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         }
                     }
                     st = ST_RAY;
+                    if (kAdj && pass == 1 && !((vis >> event) & 1ull)) {
+                        // Blocked in the forward pass: no contribution, no
+                        // gradient and no random numbers, so the continuation
+                        // starts at once instead of after an empty shadow ray.
+                        hit = true;
+                        shadow_end();
+                    }
                 }
             }
             if (path_done && kAdj) {

@@ -207,11 +207,6 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         td = 0.0f;
         hit = false;
         T.lvl = -1;
-        if (kAdj && pass == 1 && kind == RK_SHADOW && !((vis >> event) & 1ull)) {
-            hit = true;   // blocked in pass 0: no contribution, no gradient, not re-traced
-            st = ST_END;
-            return;
-        }
         rng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngTrack);
         if (kAdj && pass == 1) arng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngAdjoint);
         F3 t0, t1;
@@ -544,13 +539,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             grad_at(ev_x1, ev_w1);
         }
 
-        // ---- C0: shadow ray finished: its contribution, then the path's
-        // continuation. In the adjoint this cheap step runs every round
-        // rather than waiting for the ray-end batch: its replay ends the
-        // shadow rays that were blocked in the forward pass at once, and
-        // those lanes would otherwise idle until a batch formed (C1
-        // fwd+adjoint 100.5 -> 96.8 ms; the forward, whose shadow rays all
-        // trace, is 1.2 ms slower this way and keeps them in the batch). -----
+        // A shadow ray's end: its contribution, then the path's continuation
+        // (run in the ray-end batch below).
         auto shadow_end = [&]() {
             if (!hit) {
                 F3 le = lights_eval(K, rd);
@@ -567,7 +557,6 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             if (kAdj && pass == 1) wseg = ltot - collected;
             st = ST_RAY;
         };
-        if (kAdj && st == ST_END && kind == RK_SHADOW) shadow_end();
 
         // ---- C: ray finished (escaped or collided) -------------------------
         // (The batch shrinks with the wave's remaining lanes, so that in the
@@ -589,7 +578,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 }
                 alpha = 1.0f;
                 path_done = true;
-            } else if (!kAdj && kind == RK_SHADOW) {
+            } else if (kind == RK_SHADOW) {
                 shadow_end();
             } else if (!hit) {
                 if (depth_r > 0) {
@@ -658,7 +647,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     if (kAdj && pass == 1 && !((vis >> event) & 1ull)) {
                         // Blocked in the forward pass: no contribution, no
                         // gradient and no random numbers, so the continuation
-                        // starts at once instead of after an empty shadow ray.
+                        // starts at once instead of after an empty shadow ray
+                        // (C1 fwd+adjoint 100.5 -> 90.9 ms).
                         hit = true;
                         shadow_end();
                     }

@@ -197,6 +197,10 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     float ltot = 0.0f, collected = 0.0f, wseg = 0.0f, seg_a = 0.0f;
     uint32_t pass = 0;
     uint64_t vis = 0;
+    // The path's last contribution: the scattering event whose shadow ray
+    // escaped, or the depth of the escaping continuation (-1: none). Kept by
+    // the forward pass (record or adjoint pass 0); after it the replay stops.
+    int32_t last_c = -1;
     RayRng arng;
     arng.s0 = arng.s1 = arng.s2 = arng.s3 = 1u;
     // Gradient copy of this wave: spread over XCDs first, then over CUs.
@@ -241,6 +245,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             collected = 0.0f;
             wseg = ltot;   // the camera segment carries the whole path
         }
+        if (kRec || (kAdj && pass == 0)) last_c = -1;
         if (P.shadow_mode) {
             // RenderKernel::Shadow (src/kernel.cpp:569-607): a sun-sampled
             // shadow ray from the surface point at the compositing depth.
@@ -284,6 +289,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             adj_grad(gL, gE, gA);
             ltot = gL.x * a.x + gL.y * a.y + gL.z * a.z + gE.x * b.x + gE.y * b.y + gE.z * b.z + gA * a.w;
             vis = ((uint64_t)v.y << 32) | (uint64_t)v.x;
+            last_c = __float_as_int(b.w);
             pass = 1;
         } else {
             pass = 0;
@@ -540,14 +546,20 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         }
 
         // A shadow ray's end: its contribution, then the path's continuation
-        // (run in the ray-end batch below).
-        auto shadow_end = [&]() {
+        // (run in the ray-end batch below). Returns whether the adjoint's
+        // replay ends here: past the path's last contribution every segment
+        // carries a zero weight (what remains of the path's total), so the
+        // rest of the path has no gradient to give.
+        auto shadow_end = [&]() -> bool {
             if (!hit) {
                 F3 le = lights_eval(K, rd);
                 pathL.x += cont[6 * stride] * le.x;
                 pathL.y += cont[7 * stride] * le.y;
                 pathL.z += cont[8 * stride] * le.z;
-                if ((kAdj && pass == 0) || kRec) vis |= 1ull << event;
+                if ((kAdj && pass == 0) || kRec) {
+                    vis |= 1ull << event;
+                    last_c = (int32_t)event;
+                }
             }
             // Continue the path along the phase-sampled direction.
             rd = mk3(cont[0], cont[stride], cont[2 * stride]);
@@ -556,6 +568,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             kind = RK_CONT;
             if (kAdj && pass == 1) wseg = ltot - collected;
             st = ST_RAY;
+            return kAdj && pass == 1 && (int32_t)depth_r > last_c;
         };
 
         // ---- C: ray finished (escaped or collided) -------------------------
@@ -579,8 +592,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 alpha = 1.0f;
                 path_done = true;
             } else if (kind == RK_SHADOW) {
-                shadow_end();
+                path_done = shadow_end();
             } else if (!hit) {
+                if (depth_r > 0 && ((kAdj && pass == 0) || kRec)) last_c = (int32_t)depth_r;
                 if (depth_r > 0) {
                     F3 lc = lights_eval(K, rd);
                     F3 le = env_radiance(P.env, K, rd);
@@ -650,7 +664,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         // starts at once instead of after an empty shadow ray
                         // (C1 fwd+adjoint 100.5 -> 90.9 ms).
                         hit = true;
-                        shadow_end();
+                        path_done = shadow_end();
                     }
                 }
             }
@@ -682,7 +696,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     const size_t ri = (size_t)s_cur * W.npix + pix_local;
                     float4 *R = reinterpret_cast<float4 *>(W.rec);
                     R[ri] = make_float4(pathL.x, pathL.y, pathL.z, alpha);
-                    R[n + ri] = make_float4(pathE.x, pathE.y, pathE.z, 0.0f);
+                    R[n + ri] = make_float4(pathE.x, pathE.y, pathE.z, __int_as_float(last_c));
                     reinterpret_cast<uint2 *>(R + 2 * n)[ri] = make_uint2((uint32_t)vis, (uint32_t)(vis >> 32));
                     vis = 0;
                 }

@@ -69,7 +69,8 @@ struct WorkParams {
     uint32_t grad_nxcc;          // XCDs of the device
     // Per-sample path record, three planes over i = sample * npix + pixel
     // (samples of this launch, pixels of the rendered extent): float4
-    // {lights rgb, alpha}, float4 {env rgb, 0}, uint2 shadow-escape bits.
+    // {lights rgb, alpha}, float4 {env rgb, last contribution (int bits)},
+    // uint2 shadow-escape bits.
     // Written by kModeRecord, read by kModeAdjoint (then pass 0 is skipped).
     float *rec;
 };

@@ -206,7 +206,8 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
 /*
  * Path record (no reference counterpart): the forward render keeps, per
  * (pixel, sample) of the rendered extent, the path's lights and environment
- * radiance, its alpha and which of its shadow rays escaped (40 bytes each;
+ * radiance, its alpha, which of its shadow rays escaped and where its last
+ * contribution arose (40 bytes each;
  * drk_kernel_record_bytes gives the size of a call's record). The adjoint of
  * the same call (same tile, samples, seed, sample_base and scene densities)
  * given that record skips the replay that recomputes those totals, so it

@@ -36,6 +36,8 @@ class Batch:
         self.jobs = list(jobs)
 
     def set_tuning(self, grid_blocks=0, unit_scale=1.0):
+        """Resident workgroups (0: full occupancy) and work-unit granularity:
+        unit_scale x 16 sample chunks per pixel of every job."""
         L.check(L.lib().drk_batch_set_tuning(self._h, int(grid_blocks), float(unit_scale)))
 
     def launch(self, stream=None):

@@ -7,9 +7,10 @@
 // ended until enough of them run the ray-end code together (kEndBatch),
 // and new paths and ray starts each run as one converged pass per round.
 // A lane that finishes its work unit (one pixel x a
-// chunk of samples) refills from a unit counter with one wave-aggregated
-// atomic (ballot + mbcnt prefix), which keeps the wave's lanes busy until the
-// whole tile is drained. The tile's pixels are split into one row band per
+// chunk of samples, one sample by default) refills from its wave's reserve
+// of units, which one wave-aggregated atomic on a unit counter (ballot +
+// mbcnt prefix) tops up, so the wave's lanes stay busy until the whole tile
+// is drained. The tile's pixels are split into one row band per
 // XCD, each with its own counter: the waves of an XCD work on their own band
 // while it lasts (its rays see a smaller part of the volume, so the density
 // and gradient bricks they touch stay in that XCD's L2), then help drain the

@@ -110,6 +110,13 @@ __global__ void k_sun(KernelConsts P, const float *u1, const float *u2, const fl
     oev[3 * i] = e.x; oev[3 * i + 1] = e.y; oev[3 * i + 2] = e.z;
 }
 
+__global__ void k_map_env(const float *tex, uint32_t w, uint32_t h, const float *rd, uint32_t n, float *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    F3 c = map_env(tex, w, h, mk3(rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]));
+    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+}
+
 __global__ void k_sky(SkyState S, const float *rd, uint32_t n, float *out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -429,6 +436,21 @@ int drk_probe_camera(const float viewi[16], const float proji[16], uint32_t w, u
     PTRY(hipMemcpy(ro, dro, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
     PTRY(hipMemcpy(rd, drd, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
     PTRY(hipMemcpy(gm, dgm, 4 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
+    return DRK_OK;
+}
+
+int drk_probe_map_env(const float *tex, uint32_t w, uint32_t h, const float *rd, uint32_t n, float *out_rgb) {
+    if (!tex || !rd || !out_rgb) return set_error(DRK_EINVAL, "drk_probe_map_env: NULL argument");
+    if (w < 2 || h < 2) return set_error(DRK_EINVAL, "drk_probe_map_env: needs >= 2x2 texels");
+    if (!n) return DRK_OK;
+    DevBufs B;
+    float *dtex, *drd, *dout;
+    PTRY(B.upload(dtex, tex, 4 * (size_t)w * h), "probe upload");
+    PTRY(B.upload(drd, rd, 3 * (size_t)n), "probe upload");
+    PTRY(B.alloc(dout, 3 * (size_t)n), "probe alloc");
+    k_map_env<<<blocks(n), kProbeBlock>>>(dtex, w, h, drd, n, dout);
+    PTRY(hipGetLastError(), "probe launch");
+    PTRY(hipMemcpy(out_rgb, dout, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
     return DRK_OK;
 }
 

@@ -114,3 +114,13 @@ def camera(viewi, proji, w, h, px, py):
     L.check(L.lib().drk_probe_camera(L.fptr(_f(viewi, 16)), L.fptr(_f(proji, 16)), int(w), int(h), L.uptr(px),
                                      L.uptr(py), n, L.fptr(ro), L.fptr(rd), L.uptr(gm)))
     return ro, rd, gm
+
+
+def map_env(texels, rd):
+    """MapEnv::Evaluate then pow 2.2 on the GPU; texels (h, w, 4)."""
+    tex = _f(texels)
+    h, w = tex.shape[:2]
+    rd = _f(rd).reshape(-1, 3)
+    out = np.zeros_like(rd)
+    L.check(L.lib().drk_probe_map_env(L.fptr(tex), int(w), int(h), L.fptr(rd), len(rd), L.fptr(out)))
+    return out

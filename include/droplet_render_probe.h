@@ -43,6 +43,9 @@ int drk_probe_sun(const drk_sun *sun, const float *u1, const float *u2, const fl
  * with the sun direction of `sun` and the given turbidity / albedo. */
 int drk_probe_sky(const drk_sun *sun, float turbidity, float albedo, const float *rd, uint32_t n, float *out_rgb);
 
+/* MapEnv::Evaluate then pow 2.2 for directions rd (3n): tex is w*h RGBA texels, row-major. */
+int drk_probe_map_env(const float *tex, uint32_t w, uint32_t h, const float *rd, uint32_t n, float *out_rgb);
+
 /* Hosek state in double, as the host computes it (configs 3x9, radiances 3). */
 int drk_probe_hosek_state(double turbidity, double albedo, double elevation, double *cfg27, double *rad3);
 

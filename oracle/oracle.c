@@ -479,6 +479,10 @@ static void map_env(const float *tex, uint32_t w, uint32_t h, const float rd[3],
     }
 }
 
+void or_map_env(const float *tex, uint32_t w, uint32_t h, const float *rd, uint32_t n, float *out) {
+    for (uint32_t i = 0; i < n; ++i) map_env(tex, w, h, rd + 3 * i, out + 3 * i);
+}
+
 /* ----------------------------------------------------------- renderer -- */
 typedef struct {
     const onode *ob;

@@ -465,3 +465,14 @@ def build_octree(grid, s=None, maxd=12, sdf=None):
     if sdf is not None:
         return nodes, bricks, lvoxc, mlevel, sbricks
     return nodes, bricks, lvoxc, mlevel
+
+
+def map_env(texels, rd):
+    """MapEnv::Evaluate then pow 2.2 (src/KernelSampler.cpp:384-421, src/kernel.cpp:362-364)
+    for directions rd (n, 3); texels (h, w, 4)."""
+    tex = f32(texels)
+    h, w = tex.shape[:2]
+    rd = f32(rd).reshape(-1, 3)
+    out = np.zeros_like(rd)
+    lib().or_map_env(fp(tex), C.c_uint32(w), C.c_uint32(h), fp(rd), C.c_uint32(len(rd)), fp(out))
+    return out

@@ -28,6 +28,10 @@ DRK_MAX_LIGHTS = 8
 DRK_KERNEL_DEPTHCOMP = 0x1
 
 
+# Device-allocation gate: int gate(size_t bytes, void *user); non-zero refuses.
+ALLOC_GATE = C.CFUNCTYPE(C.c_int, C.c_size_t, C.c_void_p)
+
+
 class DrkError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("droplet-render error %d: %s" % (code, msg))
@@ -89,6 +93,10 @@ _SIGS = {
     "drk_kernel_set_seed": (C.c_int, [_P, u64, u32]),
     "drk_kernel_set_tuning": (C.c_int, [_P, u32, u32]),
     "drk_kernel_set_timing": (C.c_int, [_P, C.c_int]),
+    "drk_kernel_set_round_limit": (C.c_int, [_P, u32]),
+    "drk_kernel_limit_hit": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "drk_batch_limit_hit": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "drk_set_alloc_gate": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drk_kernel_timing": (C.c_int, [_P, C.POINTER(C.c_double), _U]),
     "drk_kernel_render": (C.c_int, [_P, u32, u32, u32, u32, u32]),
     "drk_kernel_shadow": (C.c_int, [_P, u32, u32, u32, u32, u32]),
@@ -171,3 +179,19 @@ def uptr(a):
 def dptr(a):
     assert a.dtype.name == "float64" and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_D)
+
+
+_gate_keep = None
+
+
+def set_alloc_gate(refuse):
+    """Installs drk_set_alloc_gate: refuse(bytes) -> True makes that device
+    allocation fail as out of memory. None removes the gate."""
+    global _gate_keep
+    if refuse is None:
+        check(lib().drk_set_alloc_gate(None, None))
+        _gate_keep = None
+        return
+    cb = ALLOC_GATE(lambda nbytes, user: 1 if refuse(int(nbytes)) else 0)
+    check(lib().drk_set_alloc_gate(C.cast(cb, C.c_void_p), None))
+    _gate_keep = cb

@@ -33,6 +33,18 @@ static int hip_fail(hipError_t e, const char *what) {
     return set_error(e == hipErrorOutOfMemory ? DRK_ENOMEM : DRK_EDEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Device-allocation gate (drk_set_alloc_gate): consulted before every
+// device allocation the library makes; a refusal reads as out of memory.
+static drk_alloc_gate_fn g_gate = nullptr;
+static void *g_gate_user = nullptr;
+
+template <class T>
+static hipError_t dev_malloc(T **p, size_t bytes) {
+    *p = nullptr;
+    if (g_gate && g_gate(bytes, g_gate_user) != 0) return hipErrorOutOfMemory;
+    return hipMalloc(reinterpret_cast<void **>(p), bytes);
+}
+
 #define HIP_TRY(expr, what)                      \
     do {                                         \
         hipError_t _e = (expr);                  \
@@ -99,6 +111,7 @@ struct drk_kernel {
     float *d_depth = nullptr;
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
+    uint32_t round_limit = 0;       // watchdog: rounds per wave and launch (0: none)
     uint32_t chunk_override = 0;
     uint32_t grid_override = 0;
     bool depthcomp = false;
@@ -113,7 +126,7 @@ struct drk_batch {
     std::vector<WorkParams> hW;
     RenderParams *dP = nullptr;
     WorkParams *dW = nullptr;
-    uint32_t *d_counters = nullptr;  // kMaxBands * kCounterStride words per job
+    uint32_t *d_counters = nullptr;  // kCounterWords words per job
     float *d_partial = nullptr;
     size_t partial_bytes = 0;
     uint32_t grid = 0;               // 0: occupancy-derived
@@ -130,6 +143,12 @@ struct drk_batch {
 extern "C" {
 
 const char *drk_last_error(void) { return g_err.c_str(); }
+
+int drk_set_alloc_gate(drk_alloc_gate_fn fn, void *user) {
+    g_gate = fn;
+    g_gate_user = user;
+    return DRK_OK;
+}
 
 int drk_device_info(char *name, uint32_t name_len, uint32_t *cu_count) {
     int dev = 0;
@@ -171,19 +190,25 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
         if (!(n.ce[3] > 0.0f)) return set_error(DRK_EINVAL, "drk_scene_create: node %u has non-positive extent", i);
     }
     // Interior nodes must sit above level kMaxDepth: the traversal keeps one
-    // stack word per interior level. Children have larger indices than their
-    // parents, so one forward pass assigns every level.
+    // stack word per interior level. A node may be reached through several
+    // parents (the reference's build never does this, but a hand-made or
+    // loaded array can), so its level is the deepest over all of them.
+    // Children have larger indices than their parents, so one forward pass
+    // sees every parent of a node before the node itself.
     {
-        std::vector<uint8_t> lvl(node_count, 0xff);
+        constexpr uint16_t kUnseen = 0xffff;
+        std::vector<uint16_t> lvl(node_count, kUnseen);
         lvl[0] = 0;
         for (uint32_t i = 0; i < node_count; ++i) {
-            if (lvl[i] == 0xff) continue;
+            if (lvl[i] == kUnseen) continue;
             const drk_octree_node &n = ob[i];
             bool leaf = n.volx[0] != DRK_NO_BRICK || n.volx[1] != DRK_NO_BRICK;
             if (leaf) continue;
             if (lvl[i] >= kMaxDepth) return set_error(DRK_EINVAL, "drk_scene_create: octree deeper than %u levels", kMaxDepth);
-            for (int c = 0; c < 8; ++c)
-                if (n.chn[c]) lvl[n.chn[c]] = (uint8_t)(lvl[i] + 1);
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t ch = n.chn[c];
+                if (ch && (lvl[ch] == kUnseen || lvl[ch] < lvl[i] + 1)) lvl[ch] = (uint16_t)(lvl[i] + 1);
+            }
         }
     }
     drk_scene *s = new (std::nothrow) drk_scene;
@@ -194,7 +219,7 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
     s->leafx[1] = leafx_fog;
     s->lvoxc = lvoxc;
     hipError_t e = hipGetDevice(&s->device);
-    if (e == hipSuccess) e = hipMalloc(&s->d_ob, sizeof(drk_octree_node) * node_count);
+    if (e == hipSuccess) e = dev_malloc(&s->d_ob, sizeof(drk_octree_node) * node_count);
     if (e == hipSuccess) e = hipMemcpy(s->d_ob, ob, sizeof(drk_octree_node) * node_count, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         // Child links with the child's leaf flag folded in, so a traversal
@@ -208,7 +233,7 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
                 bool leaf = m.volx[0] != DRK_NO_BRICK || m.volx[1] != DRK_NO_BRICK;
                 chx[(size_t)i * 8 + c] = ch | (leaf ? kChildLeaf : 0u);
             }
-        e = hipMalloc(&s->d_chx, chx.size() * sizeof(uint32_t));
+        e = dev_malloc(&s->d_chx, chx.size() * sizeof(uint32_t));
         if (e == hipSuccess) e = hipMemcpy(s->d_chx, chx.data(), chx.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     }
     if (e == hipSuccess && leafx_fog) {
@@ -226,14 +251,14 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
             std::memcpy(&bce[(size_t)b * 4], ob[i].ce, sizeof(ob[i].ce));
             seen[b] = 1;
         }
-        e = hipMalloc(&s->d_bce, bce.size() * sizeof(float));
+        e = dev_malloc(&s->d_bce, bce.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(s->d_bce, bce.data(), bce.size() * sizeof(float), hipMemcpyHostToDevice);
     }
     size_t nb = (size_t)leafx_fog * lvoxc * lvoxc * lvoxc;
-    if (e == hipSuccess && nb) e = hipMalloc(&s->d_fog, nb * sizeof(float));
+    if (e == hipSuccess && nb) e = dev_malloc(&s->d_fog, nb * sizeof(float));
     if (e == hipSuccess && nb) e = hipMemcpy(s->d_fog, pvol_fog, nb * sizeof(float), hipMemcpyHostToDevice);
     const size_t ns = (size_t)leafx_sdf * lvoxc * lvoxc * lvoxc;
-    if (e == hipSuccess && ns) e = hipMalloc(&s->d_sdf, ns * sizeof(float));
+    if (e == hipSuccess && ns) e = dev_malloc(&s->d_sdf, ns * sizeof(float));
     if (e == hipSuccess && ns) e = hipMemcpy(s->d_sdf, pvol_sdf, ns * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         drk_scene_destroy(s);
@@ -370,24 +395,24 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
 
     hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
     size_t tile_bytes = (size_t)tilex * tiley * 16;
-    if (e == hipSuccess) e = hipMalloc(&k->d_out, 4 * tile_bytes);
-    if (e == hipSuccess) e = hipMalloc(&k->d_counter, kMaxBands * kCounterStride * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&k->d_counts, 64);
+    if (e == hipSuccess) e = dev_malloc(&k->d_out, 4 * tile_bytes);
+    if (e == hipSuccess) e = dev_malloc(&k->d_counter, kCounterWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = dev_malloc(&k->d_counts, 64);
     if (e == hipSuccess && smp->env == DRK_ENV_MAP) {
         size_t eb = (size_t)smp->env_w * smp->env_h * 16;
-        e = hipMalloc(&k->d_env, eb);
+        e = dev_malloc(&k->d_env, eb);
         if (e == hipSuccess) e = hipMemcpy(k->d_env, smp->env_map, eb, hipMemcpyHostToDevice);
         C.env_map = k->d_env;
         C.env_w = smp->env_w;
         C.env_h = smp->env_h;
     }
-    if (e == hipSuccess) e = hipMalloc(&k->d_kc, sizeof(KernelConsts));
+    if (e == hipSuccess) e = dev_malloc(&k->d_kc, sizeof(KernelConsts));
     if (e == hipSuccess) e = hipMemcpy(k->d_kc, &C, sizeof(KernelConsts), hipMemcpyHostToDevice);
     P.kc = k->d_kc;
     // The depth map serves compositing (flag) and the Shadow pass (always).
     k->depthcomp = (flags & DRK_KERNEL_DEPTHCOMP) != 0;
     if (e == hipSuccess && pdepth) {
-        e = hipMalloc(&k->d_depth, (size_t)w * h * sizeof(float));
+        e = dev_malloc(&k->d_depth, (size_t)w * h * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(k->d_depth, pdepth, (size_t)w * h * sizeof(float), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
@@ -517,6 +542,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     WorkParams W;
     std::memset(&W, 0, sizeof(W));
     W.counter = k->d_counter;
+    W.round_limit = k->round_limit;
     W.grad_copies = 1;
     W.grad_nxcc = k->xcc;
     W.nbands = k->bands;
@@ -545,8 +571,12 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         const bool binnable = event_bins(sc->leafx[1], k->P.lvoxc3, shift, nbins) && sc->bce_ok;
         size_t budget = sc->ev_budget;
         if (budget == SIZE_MAX) {
+            // A quarter of the memory that is free now (other allocators,
+            // torch's caching one among them, may hold much of the device),
+            // plus what the current buffer already holds.
             size_t fr = 0, tot = 0;
-            budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : 0;
+            const size_t held = sc->d_ev ? (size_t)sc->ev_cap * kEventSlotBytes : 0;
+            budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (std::min(fr + held, tot)) / 4 : 0;
         }
         const uint64_t want = std::min<uint64_t>((uint64_t)npix * samples * 16, 0xffffffffull);
         const uint64_t cap = binnable ? (std::min<uint64_t>(want, budget / kEventSlotBytes) & ~63ull) : 0;
@@ -556,11 +586,23 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             sc->d_ev = nullptr;
             sc->d_evbins = nullptr;
             sc->ev_cap = 0;
-            HIP_TRY(hipMalloc(&sc->d_ev, (size_t)cap * kEventSlotBytes), "adjoint: event buffer");
-            HIP_TRY(hipMalloc(&sc->d_evbins, (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t)), "adjoint: event bins");
-            sc->ev_cap = (uint32_t)cap;
+            // Without room for events every contribution goes to the
+            // gradient with atomics instead (same gradient up to the order
+            // of float additions), so an allocation failure is not an error.
+            hipError_t e1 = dev_malloc((void **)&sc->d_ev, (size_t)cap * kEventSlotBytes);
+            hipError_t e2 = e1 == hipSuccess
+                                ? dev_malloc((void **)&sc->d_evbins, (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t))
+                                : e1;
+            if (e1 == hipSuccess && e2 == hipSuccess) {
+                sc->ev_cap = (uint32_t)cap;
+            } else {
+                (void)hipGetLastError();
+                if (sc->d_ev) (void)hipFree(sc->d_ev);
+                sc->d_ev = nullptr;
+                sc->d_evbins = nullptr;
+            }
         }
-        if (cap > 0) {
+        if (cap > 0 && sc->ev_cap > 0) {
             const uint32_t c = sc->ev_cap;
             W.ev_base = static_cast<char *>(sc->d_ev);
             W.ev_count = reinterpret_cast<unsigned long long *>(sc->d_evbins);
@@ -591,7 +633,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
                 if (k->d_gradx) (void)hipFree(k->d_gradx);
                 k->d_gradx = nullptr;
                 k->gradx_bytes = 0;
-                HIP_TRY(hipMalloc(&k->d_gradx, need), "adjoint: gradient copies");
+                HIP_TRY(dev_malloc(&k->d_gradx, need), "adjoint: gradient copies");
                 k->gradx_bytes = need;
             }
             HIP_TRY(launch_fill(k->d_gradx, (size_t)copies * gn, 0.0f, s), "adjoint: clear");
@@ -606,7 +648,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             if (k->d_partial) (void)hipFree(k->d_partial);
             k->d_partial = nullptr;
             k->partial_bytes = 0;
-            HIP_TRY(hipMalloc(&k->d_partial, need), "render: partial sums");
+            HIP_TRY(dev_malloc(&k->d_partial, need), "render: partial sums");
             k->partial_bytes = need;
         }
         W.partial = k->d_partial;
@@ -627,6 +669,17 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     HIP_TRY(launch_render(P, W, grid, mode, s, e0, e1), "render: launch");
     if (adj && copies > 1) HIP_TRY(launch_grad_sum(k->d_gradx, copies, gn, adj->grad, s), "adjoint: gradient sum");
     if (adj && E.ev_cap) HIP_TRY(launch_event_reduce(E, s), "adjoint: event reduction");
+    return DRK_OK;
+}
+
+// After a synchronous call: the status word of the kernel's last launch
+// (the stream has been synchronised), DRK_ELIMIT if a wave hit the round limit.
+static int check_limit(drk_kernel *k, hipStream_t s, const char *fn) {
+    if (k->round_limit == 0) return DRK_OK;
+    uint32_t st = 0;
+    HIP_TRY(hipMemcpyAsync(&st, k->d_counter + kStatusWord, sizeof(st), hipMemcpyDeviceToHost, s), fn);
+    HIP_TRY(hipStreamSynchronize(s), fn);
+    if (st) return set_error(DRK_ELIMIT, "%s: a wave reached the limit of %u rounds; the sums are incomplete", fn, k->round_limit);
     return DRK_OK;
 }
 
@@ -683,6 +736,8 @@ int drk_kernel_render(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
     if (rc != DRK_OK) return rc;
     HIP_TRY(hipMemcpyAsync(k->h_phb, k->d_out, 4 * tb, hipMemcpyDeviceToHost, k->stream), "drk_kernel_render: copy-out");
     HIP_TRY(hipStreamSynchronize(k->stream), "drk_kernel_render: sync");
+    rc = check_limit(k, k->stream, "drk_kernel_render");
+    if (rc != DRK_OK) return rc;
     k->sample_base += samples;
     return DRK_OK;
 }
@@ -700,7 +755,7 @@ int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, ui
     if (rc != DRK_OK) return rc;
     HIP_TRY(hipMemcpyAsync(counts, k->d_counts, 16, hipMemcpyDeviceToHost, k->stream), "drk_kernel_count: copy");
     HIP_TRY(hipStreamSynchronize(k->stream), "drk_kernel_count: sync");
-    return DRK_OK;
+    return check_limit(k, k->stream, "drk_kernel_count");
 }
 
 static int adjoint_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
@@ -749,8 +804,8 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     const size_t tb = (size_t)tilex * tiley * 16;
     const size_t gn = (size_t)k->scene->leafx[1] * k->P.lvoxc3;
     float *d_adj = nullptr, *d_grad = nullptr;
-    hipError_t e = hipMalloc(&d_adj, 2 * tb);
-    if (e == hipSuccess) e = hipMalloc(&d_grad, gn * sizeof(float));
+    hipError_t e = dev_malloc(&d_adj, 2 * tb);
+    if (e == hipSuccess) e = dev_malloc(&d_grad, gn * sizeof(float));
     if (e == hipSuccess) e = hipMemcpyAsync(d_adj, adj_lights, tb, hipMemcpyHostToDevice, k->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(reinterpret_cast<char *>(d_adj) + tb, adj_env, tb, hipMemcpyHostToDevice, k->stream);
     if (e == hipSuccess) e = launch_fill(d_grad, gn, 0.0f, k->stream);
@@ -761,6 +816,7 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
                                        k->stream);
         if (rc == DRK_OK) e = hipMemcpyAsync(grad_fog, d_grad, gn * sizeof(float), hipMemcpyDeviceToHost, k->stream);
         if (rc == DRK_OK && e == hipSuccess) e = hipStreamSynchronize(k->stream);
+        if (rc == DRK_OK && e == hipSuccess) rc = check_limit(k, k->stream, "drk_kernel_adjoint");
     }
     if (d_adj) (void)hipFree(d_adj);
     if (d_grad) (void)hipFree(d_grad);
@@ -778,6 +834,8 @@ int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
     if (rc != DRK_OK) return rc;
     HIP_TRY(hipMemcpyAsync(k->h_phb, k->d_out, 4 * tb, hipMemcpyDeviceToHost, k->stream), "drk_kernel_shadow: copy-out");
     HIP_TRY(hipStreamSynchronize(k->stream), "drk_kernel_shadow: sync");
+    rc = check_limit(k, k->stream, "drk_kernel_shadow");
+    if (rc != DRK_OK) return rc;
     k->sample_base += samples;
     return DRK_OK;
 }
@@ -785,7 +843,7 @@ int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
 int drk_kernel_footprint(const drk_kernel *k, size_t *bytes) {
     if (!k || !bytes) return set_error(DRK_EINVAL, "drk_kernel_footprint: NULL argument");
     size_t b = (size_t)4 * k->tilex * k->tiley * 16 + k->partial_bytes + k->gradx_bytes + sizeof(KernelConsts);
-    b += kMaxBands * kCounterStride * sizeof(uint32_t) + 64;
+    b += kCounterWords * sizeof(uint32_t) + 64;
     if (k->d_env) b += (size_t)k->C.env_w * k->C.env_h * 16;
     if (k->d_depth) b += (size_t)k->P.w * k->P.h * sizeof(float);
     *bytes = b;
@@ -811,6 +869,20 @@ int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes) {
     sc->d_ev = nullptr;
     sc->d_evbins = nullptr;
     sc->ev_cap = 0;
+    return DRK_OK;
+}
+
+int drk_kernel_set_round_limit(drk_kernel *k, uint32_t rounds) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_round_limit: NULL kernel");
+    k->round_limit = rounds;
+    return DRK_OK;
+}
+
+int drk_kernel_limit_hit(const drk_kernel *k, int *hit) {
+    if (!k || !hit) return set_error(DRK_EINVAL, "drk_kernel_limit_hit: NULL argument");
+    uint32_t st = 0;
+    HIP_TRY(hipMemcpy(&st, k->d_counter + kStatusWord, sizeof(st), hipMemcpyDeviceToHost), "drk_kernel_limit_hit");
+    *hit = st != 0;
     return DRK_OK;
 }
 
@@ -924,7 +996,8 @@ static int batch_plan(drk_batch *b) {
         uint32_t nfull, nchunks;
         plan_chunks(J.samples, chunk, k->chunk_override == 0, nfull, nchunks);
         if ((uint64_t)npix * nchunks > 0xf0000000ull) return set_error(DRK_EINVAL, "batch: job %u has too many work units", j);
-        W.counter = b->d_counters + (size_t)j * kMaxBands * kCounterStride;
+        W.counter = b->d_counters + (size_t)j * kCounterWords;
+        W.round_limit = k->round_limit;
         W.nbands = k->bands;
         W.npix = npix;
         W.chunk = chunk;
@@ -947,7 +1020,7 @@ static int batch_plan(drk_batch *b) {
         if (b->d_partial) (void)hipFree(b->d_partial);
         b->d_partial = nullptr;
         b->partial_bytes = 0;
-        HIP_TRY(hipMalloc(&b->d_partial, poff), "batch: partial sums");
+        HIP_TRY(dev_malloc(&b->d_partial, poff), "batch: partial sums");
         b->partial_bytes = poff;
     }
     for (uint32_t j = 0; j < nj; ++j)
@@ -979,7 +1052,7 @@ static int batch_enqueue(drk_batch *b, hipStream_t s) {
             }
         }
     }
-    HIP_TRY(launch_render_batch(b->dP, b->dW, nj, b->d_counters, nj * kMaxBands * kCounterStride, b->grid_eff, mode, s),
+    HIP_TRY(launch_render_batch(b->dP, b->dW, nj, b->d_counters, nj * kCounterWords, b->grid_eff, mode, s),
             "batch: launch");
     for (uint32_t j = 0; j < nj; ++j)
         HIP_TRY(launch_reduce(b->hW[j], b->hP[j].rx, b->hP[j].tilex, b->sumsq ? 4u : 2u, s), "batch: reduce");
@@ -1011,9 +1084,9 @@ int drk_batch_create(const drk_batch_job *jobs, uint32_t njobs, drk_batch **out)
     b->hW.resize(njobs);
     b->sumsq = sumsq;
     b->cu_count = jobs[0].kernel->cu_count;
-    hipError_t e = hipMalloc(&b->dP, sizeof(RenderParams) * njobs);
-    if (e == hipSuccess) e = hipMalloc(&b->dW, sizeof(WorkParams) * njobs);
-    if (e == hipSuccess) e = hipMalloc(&b->d_counters, sizeof(uint32_t) * njobs * kMaxBands * kCounterStride);
+    hipError_t e = dev_malloc(&b->dP, sizeof(RenderParams) * njobs);
+    if (e == hipSuccess) e = dev_malloc(&b->dW, sizeof(WorkParams) * njobs);
+    if (e == hipSuccess) e = dev_malloc(&b->d_counters, sizeof(uint32_t) * njobs * kCounterWords);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         drk_batch_destroy(b);
@@ -1112,6 +1185,17 @@ int drk_batch_autotune(drk_batch *b, void *stream, uint32_t reps, char *report, 
     log += line;
     if (report && report_len) std::snprintf(report, report_len, "%s", log.c_str());
     return batch_plan(b);
+}
+
+int drk_batch_limit_hit(const drk_batch *b, int *hit) {
+    if (!b || !hit) return set_error(DRK_EINVAL, "drk_batch_limit_hit: NULL argument");
+    const uint32_t nj = (uint32_t)b->jobs.size();
+    std::vector<uint32_t> words((size_t)nj * kCounterWords);
+    HIP_TRY(hipMemcpy(words.data(), b->d_counters, words.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+            "drk_batch_limit_hit");
+    *hit = 0;
+    for (uint32_t j = 0; j < nj; ++j) *hit |= words[(size_t)j * kCounterWords + kStatusWord] != 0;
+    return DRK_OK;
 }
 
 void drk_batch_destroy(drk_batch *b) {

@@ -364,7 +364,16 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         return -wseg * sig_e * (b - a);
     };
 
+    // Watchdog: a lane makes at most one step per round of this loop, so a
+    // limit on the wave's rounds bounds every lane's steps. A wave past the
+    // limit records it in the launch's status word and leaves; the call
+    // then reports DRK_ELIMIT (the sums of unfinished units are missing).
+    uint32_t rounds = 0;
     for (;;) {
+        if (W.round_limit != 0 && ++rounds > W.round_limit) {
+            if (lane_id() == 0) W.counter[kStatusWord] = 1u;
+            break;
+        }
         // ---- refill lanes that finished their unit ------------------------
         // The pixels are split into W.nbands contiguous row bands; a wave
         // drains the band of its own XCD first, then the others in turn.
@@ -868,7 +877,7 @@ hipError_t launch_reduce(const WorkParams &W, uint32_t rx, uint32_t tilex, uint3
 
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1) {
-    k_fill_u32<<<1, 64 * kCounterStride / 4, 0, s>>>(W.counter, W.nbands * kCounterStride, 0u);
+    k_fill_u32<<<(kCounterWords + 255) / 256, 256, 0, s>>>(W.counter, kCounterWords, 0u);
     if (ev0) (void)hipEventRecord(ev0, s);
     switch (mode) {
     case kModeForward: k_render<kModeForward><<<grid, kBlock, 0, s>>>(P, W); break;

@@ -43,8 +43,15 @@ inline void plan_chunks(uint32_t samples, uint32_t chunk, bool guided, uint32_t 
     for (uint32_t rem = samples - nfull * chunk; rem > 0; rem -= (rem + 1) / 2) ++nchunks;
 }
 constexpr uint32_t kCounterStride = 16;   // u32 words: one 64-byte line per band counter
+// Counter block of a launch: kMaxBands band counters, then one line whose
+// first word is the launch's status (1: a wave hit the round limit). The
+// whole block is cleared before every launch.
+constexpr uint32_t kStatusWord = kMaxBands * kCounterStride;
+constexpr uint32_t kCounterWords = kStatusWord + kCounterStride;
 struct WorkParams {
-    uint32_t *counter;        // per band: next unfetched unit, kCounterStride apart (reset per launch)
+    uint32_t *counter;        // counter block: per band the next unfetched unit, kCounterStride apart,
+                              // and the status word at kStatusWord (cleared per launch)
+    uint32_t round_limit;     // rounds of the state machine a wave may run in this launch (0: no limit)
     uint32_t nbands;          // row bands of the tile's pixels (1..kMaxBands)
     uint32_t total_units;     // npix * nchunks
     uint32_t npix;            // rx * ry

@@ -49,6 +49,17 @@ class RenderKernel:
     def set_tuning(self, chunk=0, grid_blocks=0):
         L.check(L.lib().drk_kernel_set_tuning(self._h, int(chunk), int(grid_blocks)))
 
+    def set_round_limit(self, rounds=0):
+        """Watchdog: rounds per wave and launch (0: none); past it the
+        synchronous calls raise DrkError(DRK_ELIMIT)."""
+        L.check(L.lib().drk_kernel_set_round_limit(self._h, int(rounds)))
+
+    def limit_hit(self):
+        """Whether the kernel's last launch stopped at the round limit."""
+        hit = C.c_int()
+        L.check(L.lib().drk_kernel_limit_hit(self._h, C.byref(hit)))
+        return bool(hit.value)
+
     def set_timing(self, enable=True):
         L.check(L.lib().drk_kernel_set_timing(self._h, 1 if enable else 0))
 

@@ -14,6 +14,7 @@
 #ifndef DROPLET_RENDER_H
 #define DROPLET_RENDER_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -218,16 +219,19 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
  */
 /*
  * Gradient events (no reference counterpart). The adjoint writes each
- * gradient contribution as a 24-byte event, sorts the events by brick and
- * sums them in on-chip memory, which avoids per-corner float atomics in HBM.
- * The event buffer belongs to the kernel's scene and is shared by the
- * scene's kernels, so adjoint calls on kernels of one scene must not overlap
- * in time. It holds 48 bytes per slot, grows to 16 slots per pixel sample of
- * the largest call so far, and is limited to `bytes` (UINT64_MAX: a quarter
- * of device memory, the default; 0: no events, every contribution is added
- * with atomics); events beyond it are added with atomics. The gradient is the
- * same either way up to the order of float additions. Setting the budget
- * frees the current buffer (after a device synchronisation).
+ * gradient contribution as a 20-byte event (weight and point; the brick in
+ * a separate plane), sorts the events by brick and sums them in on-chip
+ * memory, which avoids per-corner float atomics in HBM. The event buffer
+ * belongs to the kernel's scene and is shared by the scene's kernels, so
+ * adjoint calls on kernels of one scene must not overlap in time. It holds
+ * 40 bytes per slot (the event and its sorted copy), grows to 16 slots per
+ * pixel sample of the largest call so far, and is limited to `bytes`
+ * (UINT64_MAX, the default: a quarter of the device memory free when the
+ * buffer grows, counting the buffer itself; 0: no events, every contribution
+ * is added with atomics); events beyond it, or all of them when the buffer
+ * cannot be allocated, are added with atomics. The gradient is the same
+ * either way up to the order of float additions. Setting the budget frees
+ * the current buffer (after a device synchronisation).
  */
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes);
 
@@ -315,6 +319,35 @@ int drk_kernel_count(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, ui
  * displacement amplitude, octaves, base frequency, band width}.
  */
 int drk_synth_cloud(uint32_t kind, uint32_t n, uint64_t seed, const float *prm, uint32_t nprm, float *out);
+
+/*
+ * Watchdog (no reference counterpart; the reference's loops have no bound):
+ * limits the rounds of its state machine a wave of the path tracer may run
+ * in one launch of this kernel (0, the default: no limit). A lane takes at
+ * most one step (a traversal batch, a Woodcock step, a ray end) per round,
+ * so the limit bounds every lane's steps. A wave that reaches it stops and
+ * the launch ends normally with incomplete sums: the synchronous calls
+ * (Render, Shadow, count, the host adjoint) then return DRK_ELIMIT;
+ * drk_kernel_limit_hit reports it for the kernel's last device-buffer launch
+ * (call it after synchronising that launch's stream), drk_batch_limit_hit
+ * for a batch's last launch (jobs take their kernel's limit).
+ */
+int drk_kernel_set_round_limit(drk_kernel *k, uint32_t rounds);
+int drk_kernel_limit_hit(const drk_kernel *k, int *hit);
+int drk_batch_limit_hit(const drk_batch *b, int *hit);
+
+/*
+ * Device-allocation gate (no reference counterpart; the reference checks
+ * only RenderKernel::Initialize's buffer allocation, src/kernel.cpp:500-502):
+ * when set, gate(bytes, user) is called before every device allocation the
+ * library makes, and a non-zero return makes that allocation fail as if the
+ * device were out of memory (the call returns DRK_ENOMEM and frees what it
+ * had allocated; the adjoint's event buffer falls back to atomics). NULL
+ * removes the gate. Process-wide; set it while no other call runs. Used to
+ * test the failure paths and to cap the library's device memory.
+ */
+typedef int (*drk_alloc_gate_fn)(size_t bytes, void *user);
+int drk_set_alloc_gate(drk_alloc_gate_fn gate, void *user);
 
 /* Thread-local text of the last error. */
 const char *drk_last_error(void);

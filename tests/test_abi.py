@@ -87,6 +87,47 @@ def test_scene_create_rejects_bad_octrees_on_host(pkg):
     assert create(bad, sc.pvol[1], sc.leafx[1]) == pkg._lib.DRK_EINVAL
 
 
+def _shared_child_octree(pkg, depth):
+    """A chain 0 -> 1 -> ... -> depth-1 (slot 0) and a shortcut 0 -> s (slot 1),
+    both ending at one interior node x (child of depth-1 and of s) whose child
+    is a leaf: x sits at level `depth` through the chain, 2 through s."""
+    n = depth + 3
+    ob = np.zeros(n, pkg.Scene.node_dtype)
+    ob["volx"] = 0xFFFFFFFF
+    for i in range(n):
+        ob["ce"][i] = (0.5, 0.5, 0.5, 0.5)
+    for i in range(depth - 1):
+        ob["chn"][i, 0] = i + 1
+    s, x, leaf = depth, depth + 1, depth + 2
+    ob["chn"][0, 1] = s
+    ob["chn"][depth - 1, 0] = x
+    ob["chn"][s, 0] = x
+    ob["chn"][x, 0] = leaf
+    ob["volx"][leaf, 1] = 0
+    ob["qval"][leaf, 1] = 1.0
+    return ob
+
+
+def test_scene_create_takes_the_deepest_parent(pkg):
+    """A node shared by parents at different levels is as deep as its deepest
+    path: deeper than the traversal stack is rejected even when the parent
+    seen last in index order is shallow."""
+    L = pkg._lib.lib()
+    h = C.c_void_p()
+    brick = np.ones(512, np.float32)
+
+    def create(ob):
+        return L.drk_scene_create(ob.ctypes.data_as(C.POINTER(pkg._lib.OctreeNode)), len(ob), None, 0,
+                                  pkg._lib.fptr(brick), 1, 8, C.byref(h))
+    assert create(_shared_child_octree(pkg, 16)) == pkg._lib.DRK_EINVAL
+    assert "deeper" in L.drk_last_error().decode()
+    # shallow enough: passes the structural checks (then needs a device)
+    rc = create(_shared_child_octree(pkg, 10))
+    assert rc != pkg._lib.DRK_EINVAL
+    if rc == pkg._lib.DRK_OK:
+        L.drk_scene_destroy(h)
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, text=True)

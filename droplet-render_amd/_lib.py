@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdroplet_amd.so")
+# DRK_LIBRARY may name another build of the same library (the host-code
+# sanitizer build of build.py, loaded by the CPU tests that check it).
+LIB_PATH = os.environ.get("DRK_LIBRARY") or os.path.join(HERE, "libdroplet_amd.so")
 
 DRK_OK = 0
 DRK_EINVAL = -1

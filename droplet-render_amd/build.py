@@ -67,5 +67,36 @@ def build(verbose=False):
     return LIB
 
 
+HOST_SOURCES = [s for s in SOURCES if s.endswith(".cpp")]
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+
+
+SAN_DIR = os.path.join(OBJDIR, "sanitize")
+SAN_LIB = os.path.join(SAN_DIR, "libdroplet_amd_san.so")
+
+
+def build_sanitized():
+    """Host-code sanitizer build: the library's host sources compiled by g++
+    with AddressSanitizer and UndefinedBehaviorSanitizer and linked with the
+    device objects of the normal build (build() first), as a shared library
+    (SAN_LIB, loaded in place of the normal one when DRK_LIBRARY names it,
+    with libasan preloaded) and as the host driver tests/cpp/host_sanitize.cpp.
+    Returns the driver's path. Device code is not instrumented."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
+    objs = []
+    for src in HOST_SOURCES + [os.path.join(ROOT, "tests", "cpp", "host_sanitize.cpp")]:
+        path = src if os.path.isabs(src) else os.path.join(CSRC, src)
+        obj = os.path.join(SAN_DIR, os.path.basename(path) + ".o")
+        _run(["g++", "-std=c++17", "-O1", "-g", "-fPIC"] + SAN_FLAGS + inc + ["-c", path, "-o", obj])
+        objs.append(obj)
+    dev = [os.path.join(OBJDIR, s + ".o") for s in SOURCES if s.endswith(".hip")]
+    rocm = ["-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-lpthread"]
+    exe = os.path.join(SAN_DIR, "host_sanitize")
+    _run(["g++"] + SAN_FLAGS + ["-o", exe] + objs + dev + rocm)
+    _run(["g++", "-shared"] + SAN_FLAGS + ["-o", SAN_LIB] + objs[:-1] + dev + rocm)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -5,9 +5,10 @@
 // Layout (little endian):
 //   magic "DRKSCN\1\0" | u32 version (1) | u32 node_count | u32 leafx[2] |
 //   u32 lvoxc | u32 mlevel | f32 root_ce[4] | u64 payload bytes |
-//   u64 FNV-1a 64 of the payload |
+//   u64 FNV-1a 64 of the header's first 56 bytes and then the payload |
 //   payload: node_count 64-byte records, then leafx[0] SDF bricks and
 //   leafx[1] FOG bricks of lvoxc^3 floats each.
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,7 +19,7 @@
 namespace drk {
 
 static const char kMagic[8] = {'D', 'R', 'K', 'S', 'C', 'N', 1, 0};
-constexpr uint32_t kBlobVersion = 1;
+constexpr uint32_t kBlobVersion = 2;   // 2: the checksum covers the header fields too
 
 struct BlobHeader {
     char magic[8];
@@ -62,7 +63,8 @@ int drk_scene_save(const drk_host_scene *hs, const char *path) {
     h.mlevel = hs->mlevel;
     std::memcpy(h.root_ce, hs->root_ce, sizeof(h.root_ce));
     h.payload = nb + sb[0] + sb[1];
-    uint64_t c = fnv1a(hs->ob, nb, kFnvBasis);
+    uint64_t c = fnv1a(&h, offsetof(BlobHeader, checksum), kFnvBasis);
+    c = fnv1a(hs->ob, nb, c);
     for (int b = 0; b < 2; ++b)
         if (sb[b]) c = fnv1a(hs->pvol[b], sb[b], c);
     h.checksum = c;
@@ -129,7 +131,8 @@ int drk_scene_load(const char *path, drk_host_scene **out) {
         host_scene_free(hs);
         return set_error(DRK_EINVAL, "drk_scene_load: %s: %s", path, ok ? "trailing bytes" : "truncated payload");
     }
-    uint64_t c = fnv1a(hs->ob, nb, kFnvBasis);
+    uint64_t c = fnv1a(&h, offsetof(BlobHeader, checksum), kFnvBasis);
+    c = fnv1a(hs->ob, nb, c);
     for (int b = 0; b < 2; ++b)
         if (sb[b]) c = fnv1a(hs->pvol[b], sb[b], c);
     if (c != h.checksum) {

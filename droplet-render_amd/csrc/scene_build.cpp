@@ -232,12 +232,15 @@ int scene_build_dense(const float *fog, const float *sdf, uint32_t nx, uint32_t 
     float a = std::max(std::max(e.x, e.y), e.z);
     float lvc = 8.0f;
     float d = 2.0f * a;
-    uint32_t mlevel = (uint32_t)ceilf(logf(d / (lvc * s)) / 0.69315f);
-    if (mlevel > maxd) {
-        lvc *= powf(2.0f, (float)(mlevel - maxd));
-        mlevel = maxd;
+    // A scene smaller than one leaf gives a depth <= 0 (the reference's
+    // mlevel-1 then underflows); it is held at 1. The depth is computed as a
+    // signed integer: a negative float converted to uint32_t is undefined.
+    int32_t ml = (int32_t)ceilf(logf(d / (lvc * s)) / 0.69315f);
+    if (ml > (int32_t)maxd) {
+        lvc *= powf(2.0f, (float)(ml - (int32_t)maxd));
+        ml = (int32_t)maxd;
     }
-    if (mlevel == 0) mlevel = 1;
+    const uint32_t mlevel = ml < 1 ? 1u : (uint32_t)ml;
 
     Builder B;
     B.mlevel = mlevel;

@@ -20,7 +20,9 @@ def grids():
     g3 = np.zeros((16, 40, 24), np.float32)
     g3[0, 0, 0] = 1.0
     g3[15, 39, 23] = 0.5
-    return [("slab", g1), ("sparse", g2), ("corners", g3)]
+    g4 = np.zeros((40, 16, 24), np.float32)
+    g4[5, 3, 9] = 1.0       # smaller than half a leaf: the depth formula goes below 0 (held at 1)
+    return [("slab", g1), ("sparse", g2), ("corners", g3), ("one_voxel", g4)]
 
 
 @pytest.mark.parametrize("name,grid", grids())

@@ -105,6 +105,7 @@ _SIGS = {
     "drk_kernel_phb": (_F, [_P, u32]),
     "drk_kernel_destroy": (None, [_P]),
     "drk_kernel_render_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P, _P]),
+    "drk_kernel_render_device_noalloc": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P, _P]),
     "drk_kernel_count": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, C.POINTER(C.c_ulonglong)]),
     "drk_kernel_adjoint": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _F, _F, _F]),
     "drk_kernel_adjoint_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P]),

@@ -112,6 +112,8 @@ struct drk_kernel {
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
     uint32_t round_limit = 0;       // watchdog: rounds per wave and launch (0: none)
+    uint32_t *status = nullptr;     // counter block of the last launch (its status word at kStatusWord)
+    char *scratch = nullptr;        // this device's fixed scratch (allocation-free render), or null
     uint32_t chunk_override = 0;
     uint32_t grid_override = 0;
     bool depthcomp = false;
@@ -441,6 +443,14 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
         k->bands = (uint32_t)std::min<int>(xcc, (int)kMaxBands);
         if (const char *b = std::getenv("DRK_BANDS")) k->bands = std::max(1u, std::min<uint32_t>(std::atoi(b), kMaxBands));
     }
+    {
+        // The allocation-free render's fixed scratch: its device address,
+        // looked up once per device (here, with the kernel's device current).
+        void *p = nullptr;
+        if (scratch_address(&p) == hipSuccess) k->scratch = static_cast<char *>(p);
+        else (void)hipGetLastError();
+    }
+    k->status = k->d_counter;
     if (const char *c = std::getenv("DRK_CHUNK")) k->chunk_override = (uint32_t)std::atoi(c);
     if (const char *g = std::getenv("DRK_GRID")) k->grid_override = (uint32_t)std::atoi(g);
     *out = k;
@@ -478,7 +488,7 @@ static void rendered_extent(const drk_kernel *k, uint32_t x0, uint32_t y0, uint3
 static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
                        hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr,
-                       float *rec = nullptr) {
+                       float *rec = nullptr, bool noalloc = false) {
     if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
         return set_error(DRK_EINVAL, "render: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley, k->tilex, k->tiley);
     if (samples == 0) return set_error(DRK_EINVAL, "render: samples must be > 0");
@@ -541,7 +551,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     if ((uint64_t)npix * nchunks > 0xf0000000ull) return set_error(DRK_EINVAL, "render: too many work units");
     WorkParams W;
     std::memset(&W, 0, sizeof(W));
-    W.counter = k->d_counter;
+    W.counter = noalloc ? reinterpret_cast<uint32_t *>(k->scratch) : k->d_counter;
     W.round_limit = k->round_limit;
     W.grad_copies = 1;
     W.grad_nxcc = k->xcc;
@@ -642,7 +652,15 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             W.grad_stride = gn;
         }
     }
-    if (nchunks > 1 && !adj) {
+    if (nchunks > 1 && !adj && noalloc) {
+        const size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
+        if (need > kScratchBytes - kScratchPartialOffset)
+            return set_error(DRK_EINVAL,
+                             "render (no allocation): %u pixels x %u sample chunks need %zu bytes of partial sums; "
+                             "the fixed scratch holds %zu (use a smaller tile or fewer chunks, drk_kernel_set_tuning)",
+                             npix, nchunks, need, (size_t)(kScratchBytes - kScratchPartialOffset));
+        W.partial = reinterpret_cast<float *>(k->scratch + kScratchPartialOffset);
+    } else if (nchunks > 1 && !adj) {
         size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
         if (need > k->partial_bytes) {
             if (k->d_partial) (void)hipFree(k->d_partial);
@@ -655,7 +673,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     }
     if (count) HIP_TRY(hipMemsetAsync(k->d_counts, 0, 16, s), "render: counters");
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (k->timing && !count) {
+    if (k->timing && !count && !noalloc) {
         if (k->ev_free.empty()) {
             HIP_TRY(hipEventCreate(&e0), "render: event");
             HIP_TRY(hipEventCreate(&e1), "render: event");
@@ -666,6 +684,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         }
         k->ev_used.emplace_back(e0, e1);
     }
+    k->status = W.counter;
     HIP_TRY(launch_render(P, W, grid, mode, s, e0, e1), "render: launch");
     if (adj && copies > 1) HIP_TRY(launch_grad_sum(k->d_gradx, copies, gn, adj->grad, s), "adjoint: gradient sum");
     if (adj && E.ev_cap) HIP_TRY(launch_event_reduce(E, s), "adjoint: event reduction");
@@ -677,7 +696,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
 static int check_limit(drk_kernel *k, hipStream_t s, const char *fn) {
     if (k->round_limit == 0) return DRK_OK;
     uint32_t st = 0;
-    HIP_TRY(hipMemcpyAsync(&st, k->d_counter + kStatusWord, sizeof(st), hipMemcpyDeviceToHost, s), fn);
+    HIP_TRY(hipMemcpyAsync(&st, k->status + kStatusWord, sizeof(st), hipMemcpyDeviceToHost, s), fn);
     HIP_TRY(hipStreamSynchronize(s), fn);
     if (st) return set_error(DRK_ELIMIT, "%s: a wave reached the limit of %u rounds; the sums are incomplete", fn, k->round_limit);
     return DRK_OK;
@@ -692,6 +711,18 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t t
     hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
     return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, d_sumsq_lights,
                        d_sumsq_env, s, false);
+}
+
+int drk_kernel_render_device_noalloc(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                     uint32_t samples, uint64_t seed, uint32_t sample_base, float *d_lights,
+                                     float *d_env, float *d_sumsq_lights, float *d_sumsq_env, void *stream) {
+    if (!k || !d_lights || !d_env) return set_error(DRK_EINVAL, "drk_kernel_render_device_noalloc: NULL argument");
+    if ((d_sumsq_lights == nullptr) != (d_sumsq_env == nullptr))
+        return set_error(DRK_EINVAL, "drk_kernel_render_device_noalloc: pass both sum-of-squares buffers or neither");
+    if (!k->scratch) return set_error(DRK_EDEVICE, "drk_kernel_render_device_noalloc: no device scratch address");
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
+    return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, d_sumsq_lights,
+                       d_sumsq_env, s, false, false, nullptr, nullptr, true);
 }
 
 int drk_kernel_record_bytes(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
@@ -881,7 +912,7 @@ int drk_kernel_set_round_limit(drk_kernel *k, uint32_t rounds) {
 int drk_kernel_limit_hit(const drk_kernel *k, int *hit) {
     if (!k || !hit) return set_error(DRK_EINVAL, "drk_kernel_limit_hit: NULL argument");
     uint32_t st = 0;
-    HIP_TRY(hipMemcpy(&st, k->d_counter + kStatusWord, sizeof(st), hipMemcpyDeviceToHost), "drk_kernel_limit_hit");
+    HIP_TRY(hipMemcpy(&st, k->status + kStatusWord, sizeof(st), hipMemcpyDeviceToHost), "drk_kernel_limit_hit");
     *hit = st != 0;
     return DRK_OK;
 }

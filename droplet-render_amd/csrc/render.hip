@@ -34,6 +34,8 @@
 // record starts at pass 1.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "drk_common.h"
 #include "estimator.h"
 #include "render_launch.h"
@@ -1221,6 +1223,37 @@ hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
     const size_t lds = (size_t)E.lvoxc3 * sizeof(unsigned long long) << E.shift;
     k_ev_accum<<<acc_blocks, kEvAccBlock, lds, s>>>(E);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fixed scratch for the allocation-free render entry point: the launch's
+// counter block and its per-chunk partial sums live in this file-scope
+// device array instead of in buffers the library allocates. Every GPU the
+// library runs on holds its own copy (kScratchBytes) until the process ends.
+// In host code the array's name is not a device address: scratch_address
+// looks the address up once per device with hipGetSymbolAddress and the
+// launches use that pointer.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kScratchDevices];
+
+hipError_t scratch_address(void **ptr) {
+    *ptr = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= (int)kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {

@@ -98,6 +98,15 @@ int render_blocks_per_cu(int mode);
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+// Fixed device scratch of the allocation-free render (one per device, the
+// size below): the counter block at offset 0, partial sums from
+// kScratchPartialOffset. scratch_address gives the current device's copy
+// (looked up once per device).
+constexpr size_t kScratchBytes = 128ull << 20;
+constexpr size_t kScratchPartialOffset = 4096;
+constexpr uint32_t kScratchDevices = 64;
+hipError_t scratch_address(void **ptr);
+static_assert(kCounterWords * sizeof(uint32_t) <= kScratchPartialOffset, "counter block fits before the partials");
 // Batched forward launch over njobs (P, W) pairs in device memory; counters
 // (ncounters words, every job's band counters) are cleared first.
 int render_batch_blocks_per_cu(int mode);

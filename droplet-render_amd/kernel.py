@@ -109,6 +109,21 @@ class RenderKernel:
                                                  int(seed), int(sample_base), ptr(out_l), ptr(out_e), ptr(sq_l),
                                                  ptr(sq_e), s))
 
+    def render_device_noalloc(self, x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, sq_l=None,
+                              sq_e=None, stream=None):
+        """render_device through the allocation-free entry point (its scratch
+        is the library's fixed per-device array; calls on one GPU must not
+        overlap)."""
+        def ptr(t):
+            if t is None:
+                return None
+            assert t.is_cuda and t.dtype.is_floating_point and t.is_contiguous() and t.numel() >= tilex * tiley * 4
+            return C.c_void_p(t.data_ptr())
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_kernel_render_device_noalloc(self._h, int(x0), int(y0), int(tilex), int(tiley),
+                                                         int(samples), int(seed), int(sample_base), ptr(out_l),
+                                                         ptr(out_e), ptr(sq_l), ptr(sq_e), s))
+
     def adjoint(self, x0, y0, tilex, tiley, samples, seed, sample_base, adj_lights, adj_env):
         """dL/d(FOG brick values) for the render with these arguments, given
         dL/d(phb sums) of both buffers as (tiley, tilex, 4) arrays (host)."""

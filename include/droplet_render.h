@@ -185,6 +185,29 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
                              void *stream);
 
 /*
+ * Allocation-free form of drk_kernel_render_device (no reference
+ * counterpart): for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, work and results as
+ * drk_kernel_render_device with the same chunking, but the launch's scratch
+ * (its work-unit counters and per-chunk partial sums) lives in a fixed
+ * 128 MiB device array that the library holds on every GPU it runs on until
+ * the process ends, so the call allocates nothing. A call whose partial sums
+ * exceed that array (pixels x sample chunks x 32 bytes, 64 with sums of
+ * squares) is refused with DRK_EINVAL; a smaller tile or fewer chunks
+ * (drk_kernel_set_tuning) fits. All calls on one GPU share the array: calls
+ * that overlap in time on one GPU (other streams, other kernels, other
+ * threads) race and must be serialised by the caller. Not for depth
+ * compositing's adjoint or the path record (forward sums only).
+ */
+int drk_kernel_render_device_noalloc(drk_kernel *k, uint32_t x0, uint32_t y0,
+                                     uint32_t tilex, uint32_t tiley, uint32_t samples,
+                                     uint64_t seed, uint32_t sample_base,
+                                     float *d_lights, float *d_env,
+                                     float *d_sumsq_lights, float *d_sumsq_env,
+                                     void *stream);
+
+/*
  * Adjoint of Render (no reference counterpart; the reference is not
  * differentiable). Given dL/d(phb sums) for the lights and environment
  * buffers (tilex*tiley RGBA floats each; the alpha gradient counts once per

@@ -28,6 +28,8 @@ DRK_ENV_NULL = 1
 DRK_ENV_MAP = 2
 DRK_MAX_LIGHTS = 8
 DRK_KERNEL_DEPTHCOMP = 0x1
+DRK_ADJ_SEGMENT = 0
+DRK_ADJ_NULL_COLLISION = 1
 
 
 # Device-allocation gate: int gate(size_t bytes, void *user); non-zero refuses.
@@ -106,6 +108,9 @@ _SIGS = {
     "drk_kernel_destroy": (None, [_P]),
     "drk_kernel_render_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P, _P]),
     "drk_kernel_render_device_noalloc": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P, _P]),
+    "drk_kernel_render_weighted_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P]),
+    "drk_kernel_set_adjoint_estimator": (C.c_int, [_P, u32]),
+    "drk_scene_set_reference_fog": (C.c_int, [_P, _P]),
     "drk_kernel_count": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, C.POINTER(C.c_ulonglong)]),
     "drk_kernel_adjoint": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _F, _F, _F]),
     "drk_kernel_adjoint_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P]),

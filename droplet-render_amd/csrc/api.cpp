@@ -73,6 +73,7 @@ struct drk_scene {
     bool bce_ok = true;             // every FOG brick belongs to leaves of one ce (else no event reduction)
     float *d_fog = nullptr;
     float *d_sdf = nullptr;
+    float *d_ref = nullptr;         // reference FOG bricks of weighted tracking (or null)
     uint32_t node_count = 0;
     uint32_t leafx[2] = {0, 0};
     uint32_t lvoxc = 0;
@@ -112,6 +113,7 @@ struct drk_kernel {
     uint32_t cu_count = 0;
     uint32_t blocks_per_cu = 1;
     uint32_t round_limit = 0;       // watchdog: rounds per wave and launch (0: none)
+    uint32_t adj_estimator = DRK_ADJ_SEGMENT;   // free-flight terms of the adjoint
     uint32_t *status = nullptr;     // counter block of the last launch (its status word at kStatusWord)
     char *scratch = nullptr;        // this device's fixed scratch (allocation-free render), or null
     uint32_t chunk_override = 0;
@@ -283,6 +285,23 @@ int drk_scene_update_fog(drk_scene *s, const float *pvol_fog, const float *qval_
     return DRK_OK;
 }
 
+int drk_scene_set_reference_fog(drk_scene *s, const float *pvol_ref) {
+    if (!s) return set_error(DRK_EINVAL, "drk_scene_set_reference_fog: NULL scene");
+    if (!pvol_ref) {
+        if (s->d_ref) {
+            (void)hipDeviceSynchronize();   // a launch may still read them
+            (void)hipFree(s->d_ref);
+        }
+        s->d_ref = nullptr;
+        return DRK_OK;
+    }
+    const size_t nb = (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc;
+    if (nb == 0) return set_error(DRK_EINVAL, "drk_scene_set_reference_fog: scene has no FOG bricks");
+    if (!s->d_ref) HIP_TRY(dev_malloc(&s->d_ref, nb * sizeof(float)), "drk_scene_set_reference_fog");
+    HIP_TRY(hipMemcpy(s->d_ref, pvol_ref, nb * sizeof(float), hipMemcpyDefault), "drk_scene_set_reference_fog");
+    return DRK_OK;
+}
+
 const drk_octree_node *drk_internal_scene_nodes(const drk_scene *s) { return s ? s->d_ob : nullptr; }
 const uint32_t *drk_internal_scene_links(const drk_scene *s) { return s ? s->d_chx : nullptr; }
 
@@ -293,6 +312,7 @@ void drk_scene_destroy(drk_scene *s) {
     if (s->d_bce) (void)hipFree(s->d_bce);
     if (s->d_fog) (void)hipFree(s->d_fog);
     if (s->d_sdf) (void)hipFree(s->d_sdf);
+    if (s->d_ref) (void)hipFree(s->d_ref);
     if (s->d_ev) (void)hipFree(s->d_ev);
     if (s->d_evbins) (void)hipFree(s->d_evbins);
     delete s;
@@ -488,7 +508,7 @@ static void rendered_extent(const drk_kernel *k, uint32_t x0, uint32_t y0, uint3
 static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
                        hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr,
-                       float *rec = nullptr, bool noalloc = false) {
+                       float *rec = nullptr, bool noalloc = false, bool weighted = false) {
     if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
         return set_error(DRK_EINVAL, "render: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley, k->tilex, k->tiley);
     if (samples == 0) return set_error(DRK_EINVAL, "render: samples must be > 0");
@@ -507,7 +527,10 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     P.shadow_mode = shadow ? 1u : 0u;
     P.depth = (shadow || k->depthcomp) ? k->d_depth : nullptr;
     const bool sumsq = d_sql && d_sqe;
-    const int mode = adj ? kModeAdjoint : count ? kModeCount : rec ? kModeRecord : sumsq ? kModeForwardSq : kModeForward;
+    const int mode = adj ? (k->adj_estimator == DRK_ADJ_NULL_COLLISION ? kModeAdjointNull : kModeAdjoint)
+                     : weighted ? kModeWeighted : count ? kModeCount : rec ? kModeRecord
+                     : sumsq ? kModeForwardSq : kModeForward;
+    P.pvol_ref = weighted ? k->scene->d_ref : nullptr;
     // Clip the rendered extent to the image (gm mask, src/kernel.cpp:472-474).
     uint32_t rx, ry;
     rendered_extent(k, x0, y0, tilex, tiley, rx, ry);
@@ -725,6 +748,27 @@ int drk_kernel_render_device_noalloc(drk_kernel *k, uint32_t x0, uint32_t y0, ui
                        d_sumsq_env, s, false, false, nullptr, nullptr, true);
 }
 
+int drk_kernel_render_weighted_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                      uint32_t samples, uint64_t seed, uint32_t sample_base, float *d_lights,
+                                      float *d_env, void *stream) {
+    if (!k || !d_lights || !d_env) return set_error(DRK_EINVAL, "drk_kernel_render_weighted_device: NULL argument");
+    if (!k->scene->d_ref)
+        return set_error(DRK_EINVAL, "drk_kernel_render_weighted_device: the scene has no reference densities");
+    if (k->scene->leafx[0] != 0)
+        return set_error(DRK_EINVAL, "drk_kernel_render_weighted_device: scenes with SDF leaves are not supported");
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
+    return render_impl(k, x0, y0, tilex, tiley, samples, seed, sample_base, d_lights, d_env, nullptr, nullptr, s,
+                       false, false, nullptr, nullptr, false, true);
+}
+
+int drk_kernel_set_adjoint_estimator(drk_kernel *k, uint32_t estimator) {
+    if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_adjoint_estimator: NULL kernel");
+    if (estimator != DRK_ADJ_SEGMENT && estimator != DRK_ADJ_NULL_COLLISION)
+        return set_error(DRK_EINVAL, "drk_kernel_set_adjoint_estimator: unknown estimator %u", estimator);
+    k->adj_estimator = estimator;
+    return DRK_OK;
+}
+
 int drk_kernel_record_bytes(const drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
                             uint32_t samples, size_t *bytes) {
     if (!k || !bytes) return set_error(DRK_EINVAL, "drk_kernel_record_bytes: NULL argument");
@@ -887,6 +931,7 @@ int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
              ((size_t)s->leafx[0] + s->leafx[1]) * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float) +
              (s->d_bce ? (size_t)s->leafx[1] * 4 * sizeof(float) : 0);
     if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
+    if (s->d_ref) *bytes += (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
     return DRK_OK;
 }
 

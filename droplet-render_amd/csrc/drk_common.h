@@ -111,6 +111,7 @@ struct RenderParams {
     const uint32_t *chx;      // node_count x 8 child links (index | kChildLeaf)
     const float *pvol_fog;
     const float *pvol_sdf;
+    const float *pvol_ref;    // weighted tracking: the reference FOG bricks that decide collisions
     const float *depth;       // w*h compositing depth or nullptr
     const KernelConsts *kc;
     uint32_t lvoxc, lvoxc3;

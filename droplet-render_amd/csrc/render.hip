@@ -150,8 +150,10 @@ template <int kMode>
 __device__ __forceinline__ void render_body(const RenderParams &P, const WorkParams &W, RenderLds &S) {
     constexpr bool kSumSq = (kMode == kModeForwardSq || kMode == kModeCount);
     constexpr bool kCount = (kMode == kModeCount);
-    constexpr bool kAdj = (kMode == kModeAdjoint);
+    constexpr bool kNull = (kMode == kModeAdjointNull);   // adjoint with null-collision terms
+    constexpr bool kAdj = (kMode == kModeAdjoint) || kNull;
     constexpr bool kRec = (kMode == kModeRecord);
+    constexpr bool kWt = (kMode == kModeWeighted);       // weighted delta tracking
     MieLds &mie = S.mie;
     unsigned long long (*ev_cur)[2] = S.ev_cur;
 
@@ -184,6 +186,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     // Ray.
     F3 ro = mk3(0, 0, 0), rd = mk3(0, 0, 1);
     bool hit = false;
+    // Weighted tracking: the ray's weight, the product of its collisions'
+    // density ratios (exactly 1 where the densities equal the reference).
+    float rw = 1.0f;
     Traversal T;
     T.lvl = -1;
     RayRng rng;
@@ -214,8 +219,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         td = 0.0f;
         hit = false;
         T.lvl = -1;
+        if (kWt) rw = 1.0f;
         rng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngTrack);
-        if (kAdj && pass == 1) arng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngAdjoint);
+        if (kAdj && !kNull && pass == 1) arng.seed(key0, P.key1, P.sample_base + s_cur, ray_id, kRngAdjoint);
         F3 t0, t1;
         uint32_t a;
         if (!octree_init(P.ob, ro, rd, t0, t1, a) || t1.x < 0.0f || t1.y < 0.0f || t1.z < 0.0f) {
@@ -520,7 +526,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             if (!(sc < tr1)) {
                 td = trb;     // left the leaf: memoryless restart in the next one
                 st = ST_TRAV;
-                if (kAdj && pass == 1 && kmaj > 0.0f) ev_w0 = grad_segment(seg_a, trb, ev_x0);
+                if (kAdj && !kNull && pass == 1 && kmaj > 0.0f) ev_w0 = grad_segment(seg_a, trb, ev_x0);
             } else if (!(sc > tr0)) {
                 hit = true;   // collision before the leaf entry counts as a real one (src/kernel.cpp:287)
                 st = ST_END;
@@ -541,13 +547,26 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     if (kCount) cnt_lookups += 1;
                 }
                 float q = rng.uniform();
-                if (!(p / smax < q) || !(d > 0.0f)) {
+                // Weighted tracking: the reference density decides, with the
+                // same comparison; the ray's weight takes the ratio of the
+                // probabilities of that decision under the two densities
+                // (fog-only leaves: the weighted entry point refuses SDF).
+                const float pd = kWt ? brick_lookup(P.pvol_ref + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc) : p;
+                if (!(pd / smax < q) || !(d > 0.0f)) {
                     hit = true;
                     st = ST_END;
+                    if (kWt) rw *= p / pd;
                     if (kAdj && pass == 1) {
-                        ev_w0 = grad_segment(seg_a, td, ev_x0);
+                        if (!kNull) ev_w0 = grad_segment(seg_a, td, ev_x0);
                         ev_x1 = rc;
                         ev_w1 = wseg / p;   // d sigma / sigma at the real collision
+                    }
+                } else {
+                    if (kWt) rw *= (smax - p) / (smax - pd);
+                    if (kNull && pass == 1) {
+                        // d/d rho of the null collision's probability 1 - rho/smax, relative
+                        ev_x0 = rc;
+                        ev_w0 = -wseg / (smax - p);
                     }
                 }
             }
@@ -565,6 +584,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         auto shadow_end = [&]() -> bool {
             if (!hit) {
                 F3 le = lights_eval(K, rd);
+                if (kWt) le = mk3(le.x * rw, le.y * rw, le.z * rw);
                 pathL.x += cont[6 * stride] * le.x;
                 pathL.y += cont[7 * stride] * le.y;
                 pathL.z += cont[8 * stride] * le.z;
@@ -593,6 +613,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                              !__ballot(st == ST_TRAV || st == ST_LEAF);
         if (run_end && st == ST_END) {
             bool path_done = false;
+            // Weighted tracking: a camera or continuation ray's weight joins
+            // the path throughput before the ray's end uses it.
+            if (kWt && kind != RK_SHADOW && kind != RK_SHADOW_PASS) beta = mk3(beta.x * rw, beta.y * rw, beta.z * rw);
             if (kind == RK_SHADOW_PASS) {
                 // SampleVolume at r = scattevs returns the lights' radiance when
                 // the ray escapes (r > 0); Shadow divides by lights[0]'s colour.
@@ -616,7 +639,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 path_done = true;
             } else {
                 if (depth_r == 0) {
-                    alpha = 1.0f;
+                    alpha = kWt ? rw : 1.0f;
                     if (kAdj && pass == 1) {
                         F3 gL, gE;
                         float gA;
@@ -785,6 +808,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     render_body<kModeAdjoint>(P, W, S);
 }
 
+// The adjoint with null-collision terms (the pathwise derivative of the
+// weighted forward), same occupancy as the default adjoint.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint_null(RenderParams P,
+                                                                                                         WorkParams W) {
+    __shared__ RenderLds S;
+    render_lds_init(S);
+    render_body<kModeAdjointNull>(P, W, S);
+}
+
 // Many launches (scenes, views, tiles) in one persistent grid. Workgroup b
 // starts at job b % njobs and visits every job once; each of its waves drains
 // the job's work units (shared with every other workgroup on that job) before
@@ -844,6 +876,8 @@ int render_blocks_per_cu(int mode) {
     case kModeForwardSq: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeForwardSq>, kBlock, 0); break;
     case kModeCount: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeCount>, kBlock, 0); break;
     case kModeRecord: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeRecord>, kBlock, 0); break;
+    case kModeWeighted: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render<kModeWeighted>, kBlock, 0); break;
+    case kModeAdjointNull: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_adjoint_null, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_adjoint, kBlock, 0); break;
     }
     if (e != hipSuccess || nb <= 0) return 1;
@@ -886,12 +920,14 @@ hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t gr
     case kModeForwardSq: k_render<kModeForwardSq><<<grid, kBlock, 0, s>>>(P, W); break;
     case kModeCount: k_render<kModeCount><<<grid, kBlock, 0, s>>>(P, W); break;
     case kModeRecord: k_render<kModeRecord><<<grid, kBlock, 0, s>>>(P, W); break;
+    case kModeWeighted: k_render<kModeWeighted><<<grid, kBlock, 0, s>>>(P, W); break;
+    case kModeAdjointNull: k_render_adjoint_null<<<grid, kBlock, 0, s>>>(P, W); break;
     default: k_render_adjoint<<<grid, kBlock, 0, s>>>(P, W); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev1) (void)hipEventRecord(ev1, s);
-    if (W.partial && mode != kModeAdjoint) {
+    if (W.partial && mode != kModeAdjoint && mode != kModeAdjointNull) {
         uint32_t nb = (W.npix + 255) / 256;
         bool sq = (mode == kModeForwardSq || mode == kModeCount);
         k_reduce<<<nb, 256, 0, s>>>(W, P.rx, P.tilex, sq ? 4u : 2u);

@@ -88,6 +88,8 @@ constexpr int kModeForwardSq = 1;   // sums and sums of squares
 constexpr int kModeCount = 2;       // sums, squares and work counters
 constexpr int kModeAdjoint = 3;     // density gradients by path replay
 constexpr int kModeRecord = 4;      // sums plus the per-sample path record
+constexpr int kModeWeighted = 5;    // sums of weighted delta tracking against reference densities
+constexpr int kModeAdjointNull = 6; // density gradients by path replay, null-collision terms
 
 // Bytes of one (pixel, sample) entry of the path record.
 constexpr size_t kRecordBytes = 40;

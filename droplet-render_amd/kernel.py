@@ -124,6 +124,21 @@ class RenderKernel:
                                                          int(samples), int(seed), int(sample_base), ptr(out_l),
                                                          ptr(out_e), ptr(sq_l), ptr(sq_e), s))
 
+    def render_weighted_device(self, x0, y0, tilex, tiley, samples, seed, sample_base, out_l, out_e, stream=None):
+        """Weighted delta tracking against the scene's reference densities
+        (Scene.set_reference_fog) into torch tensors (asynchronous)."""
+        for t in (out_l, out_e):
+            assert t.is_cuda and t.dtype.is_floating_point and t.is_contiguous() and t.numel() >= tilex * tiley * 4
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_kernel_render_weighted_device(self._h, int(x0), int(y0), int(tilex), int(tiley),
+                                                          int(samples), int(seed), int(sample_base),
+                                                          C.c_void_p(out_l.data_ptr()), C.c_void_p(out_e.data_ptr()),
+                                                          s))
+
+    def set_adjoint_estimator(self, estimator):
+        """DRK_ADJ_SEGMENT (default) or DRK_ADJ_NULL_COLLISION."""
+        L.check(L.lib().drk_kernel_set_adjoint_estimator(self._h, int(estimator)))
+
     def adjoint(self, x0, y0, tilex, tiley, samples, seed, sample_base, adj_lights, adj_env):
         """dL/d(FOG brick values) for the render with these arguments, given
         dL/d(phb sums) of both buffers as (tiley, tilex, 4) arrays (host)."""

@@ -125,6 +125,21 @@ class Scene:
         if q is not None:
             self.ob["qval"][:, L.DRK_VOLUME_FOG] = q
 
+    def set_reference_fog(self, pvol_ref):
+        """Reference FOG bricks for weighted tracking (host array or torch
+        tensor of leafx*lvoxc^3 floats); None drops them."""
+        if pvol_ref is None:
+            L.check(L.lib().drk_scene_set_reference_fog(self.device(), None))
+            return
+        if hasattr(pvol_ref, "data_ptr"):
+            assert pvol_ref.numel() == self.leafx[1] * self.lvoxc3
+            ptr = C.c_void_p(pvol_ref.data_ptr())
+        else:
+            self._ref = np.ascontiguousarray(pvol_ref, dtype=np.float32).reshape(-1)
+            assert self._ref.size == self.leafx[1] * self.lvoxc3
+            ptr = C.c_void_p(self._ref.ctypes.data)
+        L.check(L.lib().drk_scene_set_reference_fog(self.device(), ptr))
+
     def raise_majorants(self, factor=1.25, floor=None):
         """Multiplies every non-zero FOG majorant by `factor` (any value at or
         above the brick maximum keeps delta tracking unbiased); used to keep

@@ -228,6 +228,39 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
                        float *grad_fog);
 
 /*
+ * Estimators of the adjoint's free-flight terms (no reference counterpart).
+ * DRK_ADJ_SEGMENT (the default): differential delta tracking, one uniform
+ * point per covered leaf interval, -W sigma_e (b-a) w_v(point).
+ * DRK_ADJ_NULL_COLLISION: one term per null collision x of the replayed
+ * ray, -W w_v(x) / (majorant - rho(x)): the exact derivative, path by path,
+ * of drk_kernel_render_weighted_device at its reference densities, so
+ * central differences of that forward with common random numbers equal it
+ * up to O(h^2). Both are unbiased for the derivative of the expected image.
+ */
+#define DRK_ADJ_SEGMENT        0
+#define DRK_ADJ_NULL_COLLISION 1
+int drk_kernel_set_adjoint_estimator(drk_kernel *k, uint32_t estimator);
+
+/*
+ * Weighted delta tracking (no reference counterpart): a density-smooth
+ * forward for finite-difference checks of the adjoint. The scene's
+ * reference FOG bricks (drk_scene_set_reference_fog: leafx*lvoxc^3 floats,
+ * host or device memory; NULL drops them) decide every tentative collision
+ * with the reference's comparison, and each ray carries the ratio of that
+ * decision's probabilities under the current and the reference densities
+ * (rho/rho_ref when accepted, (q-rho)/(q-rho_ref) when rejected, q the
+ * leaf's majorant). Its expected image equals the plain render's for the
+ * current densities (at or below the majorants); where they equal the
+ * reference the image is the plain render's bit for bit, and it is smooth
+ * in the densities under common random numbers. Sums only (no alpha
+ * counts: alpha is the weighted scatter count). FOG-only scenes.
+ */
+int drk_scene_set_reference_fog(drk_scene *scene, const float *pvol_ref);
+int drk_kernel_render_weighted_device(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley,
+                                      uint32_t samples, uint64_t seed, uint32_t sample_base,
+                                      float *d_lights, float *d_env, void *stream);
+
+/*
  * Path record (no reference counterpart): the forward render keeps, per
  * (pixel, sample) of the rendered extent, the path's lights and environment
  * radiance, its alpha, which of its shadow rays escaped and where its last

@@ -21,6 +21,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz peak engine
+# clock = 39.3 T lane-operations per second (one plain FP32 operation per
+# lane per cycle; wave64 instructions take 4 cycles on a 16-lane SIMD).
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+PMC_PASSES = [
+    # FETCH_SIZE uses 3 TCC counters; 4 SQ and 1 GRBM counter share the pass
+    ["FETCH_SIZE", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE"],
+    # WRITE_SIZE uses 2 TCC counters, the L2 hit/miss pair the other 2
+    ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"],
+]
 METRIC = "delta-tracking Msamples/s fwd & fwd+adjoint vs plain CPU, 256³ cloud, 1 GPU"
 
 
@@ -31,13 +41,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline wall time")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline wall time per build")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 1 GiB device copies (a kernel of known traffic, for scaling the "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
+    ap.add_argument("--pmc-child", action="store_true",
+                    help="internal: the program rocprofv3 runs for the counter passes (copies + one forward)")
+    ap.add_argument("--sample-base", type=int, default=0)
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--pmc-report", default=None, help="write the counter summary to this file")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -49,7 +64,7 @@ def main():
     import torch
     from __graft_entry__ import load_package
 
-    if args.copy_calibration:
+    if args.copy_calibration or args.pmc_child:
         n = 1 << 28   # floats: 1 GiB per copy
         a = torch.ones(n, device="cuda")
         b = torch.empty_like(a)
@@ -57,11 +72,25 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
-        return
+        del a, b
+        if args.copy_calibration:
+            return
 
     pkg = load_package()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+
+    if args.pmc_child:
+        # One forward of the benchmark's workload, as a timed step renders it.
+        wl = getattr(pkg.configs, args.workload)()
+        k = wl.kernel()
+        npix = wl.w * wl.h
+        out_l = torch.empty(npix * 4, dtype=torch.float32, device="cuda")
+        out_e = torch.empty(npix * 4, dtype=torch.float32, device="cuda")
+        k.render_device(0, 0, wl.w, wl.h, wl.spp, args.seed, args.sample_base, out_l, out_e)
+        torch.cuda.synchronize()
+        k.Destroy()
+        return
 
     if args.workload == "c2":
         bench_batch(pkg, torch, args)
@@ -111,12 +140,7 @@ def main():
     n_node /= args.steps
     n_look /= args.steps
     alg_bytes = 64.0 * n_node + 32.0 * n_look + 32.0 * npix
-    achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s.json" % wl.name)
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    pmc = None if args.no_pmc else measure_pmc(args, args.warmup * wl.spp)
 
     # Forward + adjoint: one step = the forward render (keeping the per-sample
     # path record) plus the path-replay adjoint of the same samples for a
@@ -165,6 +189,10 @@ def main():
                    "no_record_ratio_to_forward": round((tr / args.adj_steps) / fwd_ms, 3),
                    "record_bytes": rec_bytes,
                    "kernel_device_bytes": k.footprint(),
+                   "device_bytes": k.footprint() + wl.scene.footprint() + rec_bytes + 4 * grad.numel() +
+                   4 * adj_l.numel() + 4 * adj_e.numel(),
+                   "device_bytes_note": "kernel buffers + scene (nodes, bricks, the adjoint's event buffer) + path "
+                                        "record + gradient + image gradients",
                    "gradient_finite": bool(torch.isfinite(grad).all().item())}
         del rec
 
@@ -176,19 +204,36 @@ def main():
         threads = max(1, min(threads, os.cpu_count() or 1))
         # Bounded sample: the full frame at a reduced number of paths per
         # pixel (sample indices 0..spp_cpu-1 of every pixel), sized from a
-        # 1-spp probe to about --cpu-seconds of work.
-        tp = time.perf_counter()
-        O.render(wl, seed=args.seed, samples=1, threads=threads)
-        tp = time.perf_counter() - tp
-        spp_cpu = int(max(1, min(wl.spp, args.cpu_seconds / max(tp, 1e-3))))
-        tc = time.perf_counter()
-        ref = O.render(wl, seed=args.seed, samples=spp_cpu, threads=threads)
-        tc = time.perf_counter() - tc
-        cpu = {"value": round(npix * spp_cpu / tc / 1e6, 4), "unit": "Msamples/s", "cores": threads,
-               "kind": "port",
-               "sample": "CPU oracle (oracle/oracle.c), full %dx%d frame at %d of %d paths per pixel "
-                         "(%d paths, %.1f s), same scene/camera/seed" % (wl.w, wl.h, spp_cpu, wl.spp,
-                                                                        npix * spp_cpu, tc)}
+        # 1-spp probe to about --cpu-seconds of work, for each build of the
+        # restatement: the -O2 scalar one the tests use and an -O3
+        # x86-64-v3 build of the same code (identical numbers).
+        variants = []
+        ref = None
+        for fast in (False, True):
+            tp = time.perf_counter()
+            O.render(wl, seed=args.seed, samples=1, threads=threads, fast=fast)
+            tp = time.perf_counter() - tp
+            spp_cpu = int(max(1, min(wl.spp, args.cpu_seconds / max(tp, 1e-3))))
+            tc = time.perf_counter()
+            r = O.render(wl, seed=args.seed, samples=spp_cpu, threads=threads, fast=fast)
+            tc = time.perf_counter() - tc
+            variants.append({"build": "-O3 -march=x86-64-v3" if fast else "-O2", "spp": spp_cpu,
+                             "seconds": round(tc, 2), "value": round(npix * spp_cpu / tc / 1e6, 4)})
+            if ref is None or spp_cpu < ref[1]:
+                ref = (r, spp_cpu)
+        best = max(variants, key=lambda v: v["value"])
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            pass
+        cpu = {"value": best["value"], "unit": "Msamples/s", "cores": threads, "kind": "port",
+               "sample": "CPU oracle (oracle/oracle.c, one lane per path on %d threads), full %dx%d frame at %d of "
+                         "%d paths per pixel (%.1f s), same scene/camera/seed; fastest of the builds listed"
+                         % (threads, wl.w, wl.h, best["spp"], wl.spp, best["seconds"]),
+               "builds": variants, "cpu_model": model, "nproc": os.cpu_count()}
+        ref, spp_cpu = ref
         # Same sample on the GPU (same RNG keys) for a path-level parity check.
         k.set_seed(args.seed, 0)
         k.Render(0, 0, wl.w, wl.h, spp_cpu)
@@ -226,23 +271,142 @@ def main():
             "brick_occupancy": round(wl.scene.occupancy(), 4), "scene_build_s": round(t_build, 3),
             "device": name.value.decode(), "cus": cu.value,
         },
-        "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "kernel": "k_render", "kernel_ms": round(kern_avg_ms, 4),
-            "alg_bytes_per_launch": int(alg_bytes), "node_reads_per_launch": int(n_node),
-            "brick_lookups_per_launch": int(n_look),
-            "hbm_measured_gbs": round(traffic / (kern_avg_ms * 1e6), 1) if traffic else None,
-            "note": "algorithmic bytes of the reference layout (64 B per node record the walk visits, 32 B per "
-                    "trilinear lookup, 32 B per pixel); they are served from L2/MALL, so achieved can exceed the "
-                    "HBM peak; the HBM bytes actually moved are 'traffic' (PMC FETCH_SIZE + WRITE_SIZE)",
-        },
+        "roofline": roofline(pmc, kern_avg_ms, alg_bytes, n_node, n_look),
         "fwd_adjoint": fwd_adj,
         "cpu_baseline": cpu,
         "parity": parity,
     }
     k.Destroy()
     print(json.dumps(line), flush=True)
+
+
+def _pmc_run(args, counters, sample_base, outdir):
+    """One rocprofv3 counter pass over `bench.py --pmc-child` (three 1 GiB
+    copies, then one forward of the workload); returns {kernel: {counter:
+    mean value per dispatch}} from its CSV, or raises."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        raise RuntimeError("rocprofv3 not found")
+    cmd = [exe, "--pmc"] + counters + ["-d", outdir, "-o", "run", "--output-format", "csv", "--",
+                                       sys.executable, os.path.abspath(__file__), "--pmc-child",
+                                       "--workload", args.workload, "--seed", str(args.seed),
+                                       "--sample-base", str(sample_base)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         start_new_session=True)
+    try:
+        out, _ = p.communicate(timeout=150)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        raise RuntimeError("rocprofv3 pass timed out: %s" % " ".join(counters))
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("no counter file (rc %d): %s" % (p.returncode, out.decode(errors="replace")[-400:]))
+    per = {}
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                key = (row.get("Kernel_Name", ""), row.get("Dispatch_Id", ""))
+                per.setdefault(key, {})
+                c = row["Counter_Name"]
+                per[key][c] = per[key].get(c, 0.0) + float(row["Counter_Value"])
+    return [(name, vals) for (name, _), vals in per.items()]
+
+
+def _pick(dispatches, pattern, largest=None):
+    """Mean counters per dispatch of the kernels whose name matches; with
+    `largest`, only the dispatches within 10% of the largest value of that
+    counter (the 1 GiB copies, not the small copies of the scene upload)."""
+    import re
+    hits = [v for n, v in dispatches if re.search(pattern, n)]
+    if largest:
+        top = max(v.get(largest, 0.0) for v in hits) if hits else 0.0
+        hits = [v for v in hits if v.get(largest, 0.0) >= 0.9 * top]
+    if not hits:
+        return None
+    return {c: sum(v.get(c, 0.0) for v in hits) / len(hits) for c in hits[0]}
+
+
+def measure_pmc(args, sample_base):
+    """HBM traffic, L2 hit rate and VALU counters of the path-tracing kernel
+    for the benchmark's workload, measured now by two rocprofv3 --pmc passes
+    of a child process (FETCH_SIZE and WRITE_SIZE scaled by what they read
+    for a copy of known size in the same process). Returns a dict, with
+    'error' set if the passes could not run."""
+    import tempfile
+    res = {"passes": [" ".join(c) for c in PMC_PASSES]}
+    try:
+        vals = {}
+        copy = {}
+        for i, counters in enumerate(PMC_PASSES):
+            with tempfile.TemporaryDirectory(prefix="drk_pmc_", dir="/tmp") as d:
+                agg = _pmc_run(args, counters, sample_base, d)
+            k = _pick(agg, r"k_render<0>|k_renderILi0E")
+            c = _pick(agg, r"(?i)copy", largest=counters[0])
+            if k is None or c is None:
+                raise RuntimeError("kernels not found in pass %d: %s" % (i, sorted({n for n, _ in agg})[:6]))
+            vals.update(k)
+            copy.update(c)
+        gib_kib = float(1 << 20)
+        ff = gib_kib / copy["FETCH_SIZE"]
+        wf = gib_kib / copy["WRITE_SIZE"]
+        fetch = vals["FETCH_SIZE"] * 1024.0 * ff
+        write = vals["WRITE_SIZE"] * 1024.0 * wf
+        res.update({
+            "fetch_bytes": fetch, "write_bytes": write, "fetch_factor": ff, "write_factor": wf,
+            "l2_hit_rate": vals["TCC_HIT_sum"] / max(vals["TCC_HIT_sum"] + vals["TCC_MISS_sum"], 1.0),
+            "valu_lane_ops": vals["SQ_THREAD_CYCLES_VALU"], "valu_insts": vals["SQ_INSTS_VALU"],
+            "valu_active_insts": vals["SQ_ACTIVE_INST_VALU"], "waves": vals["SQ_WAVES"],
+            "gui_active_cycles": vals["GRBM_GUI_ACTIVE"], "raw": vals, "copy_raw": copy,
+        })
+    except Exception as e:   # the benchmark line still prints; the roofline says why it lacks counters
+        res["error"] = str(e)[:300]
+    if args.pmc_report:
+        with open(args.pmc_report, "w") as f:
+            json.dump(res, f, indent=1)
+    return res
+
+
+def roofline(pmc, kern_ms, alg_bytes, n_node, n_look):
+    """The path-tracing kernel's roofline: VALU lane-operations (the bound:
+    divergent per-lane state machines saturate VALU issue with part of the
+    lanes active; the working set is served from L2/MALL), with the HBM
+    traffic and the algorithmic bytes beside it."""
+    sec = kern_ms * 1e-3
+    alg = {"bytes_per_launch": int(alg_bytes), "node_reads_per_launch": int(n_node),
+           "brick_lookups_per_launch": int(n_look), "gbs": round(alg_bytes / sec / 1e9, 1),
+           "note": "cache-served, not an HBM fraction: 64 B per octree node record the walk visits, 32 B per "
+                   "trilinear lookup, 32 B per pixel; the octree and bricks stay in L2/MALL"}
+    line = {"bound": "valu", "achieved": None, "peak": round(VALU_PEAK_LANE_OPS / 1e12, 2), "unit": "T lane-op/s",
+            "frac": None, "traffic": None, "kernel": "k_render<0>", "kernel_ms": round(kern_ms, 4),
+            "peak_note": "256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (plain FP32 lane-ops)",
+            "algorithmic_bytes": alg}
+    if pmc and "error" not in pmc:
+        ops = pmc["valu_lane_ops"]
+        traffic = pmc["fetch_bytes"] + pmc["write_bytes"]
+        line.update({
+            "achieved": round(ops / sec / 1e12, 3), "frac": round(ops / sec / VALU_PEAK_LANE_OPS, 4),
+            "traffic": int(traffic),
+            "lane_ops_per_launch": int(ops),
+            "lane_utilisation": round(ops / (pmc["valu_active_insts"] * 64.0), 4),
+            "valu_issue_per_simd_cycle": round(pmc["valu_insts"] / (1024.0 * pmc["gui_active_cycles"] / 8.0), 4),
+            "hbm": {"bytes_per_launch": int(traffic), "fetch": int(pmc["fetch_bytes"]),
+                    "write": int(pmc["write_bytes"]), "gbs": round(traffic / sec / 1e9, 1),
+                    "frac_of_8TBs": round(traffic / sec / 1e9 / HBM_PEAK_GBS, 4),
+                    "scale": "FETCH_SIZE x %.3f, WRITE_SIZE x %.3f (copy of 1 GiB)" % (pmc["fetch_factor"],
+                                                                                      pmc["write_factor"])},
+            "l2_hit_rate": round(pmc["l2_hit_rate"], 4),
+            "counters": "rocprofv3 --pmc, measured in this run: " + "; ".join(pmc["passes"]),
+        })
+    elif pmc:
+        line["counters_error"] = pmc["error"]
+    return line
 
 
 def bench_batch(pkg, torch, args):

@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "_build", "liboracle.so")
+LIB_FAST = os.path.join(HERE, "_build", "liboracle_fast.so")   # -O3 -march=x86-64-v3 build, same numbers
 REF_HOSEK = os.path.join(HERE, "_ref", "libarhosek_ref.so")
 
 sys.path.insert(0, os.path.join(ROOT, "droplet-render_amd", "data"))
@@ -47,22 +48,22 @@ class OParams(C.Structure):
                 ("mie", C.POINTER(OMie)), ("depth", _F), ("key1", C.c_uint32)]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(fast=False):
+    path = LIB_FAST if fast else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             raise ImportError("oracle not built: make -C oracle")
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         assert L.or_params_size() == C.sizeof(OParams), (L.or_params_size(), C.sizeof(OParams))
         assert L.or_mie_size() == C.sizeof(OMie)
         L.or_mie_theta.restype = C.c_float
         L.or_voxel.restype = C.c_float
         L.or_hg_eval.restype = C.c_float
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def f32(a):
@@ -256,16 +257,18 @@ def make_params(wl, seed, depth=None):
     return P, keep
 
 
-def render(wl, seed, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None, threads=None, depth=None):
-    """CPU render of a tile; returns [lights, env, sumsq lights, sumsq env] as (tiley, tilex, 4) float32."""
+def render(wl, seed, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None, threads=None, depth=None,
+           fast=False):
+    """CPU render of a tile; returns [lights, env, sumsq lights, sumsq env] as (tiley, tilex, 4) float32.
+    fast: the -O3 x86-64-v3 build of the same restatement."""
     tilex = tilex or wl.w
     tiley = tiley or wl.h
     samples = samples or wl.spp
     threads = threads or os.cpu_count() or 1
     P, keep = make_params(wl, seed, depth)
     out = np.zeros((4, tiley, tilex, 4), np.float32)
-    lib().or_render(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
-                    C.c_uint32(samples), C.c_uint32(sample_base), C.c_uint32(threads), fp(out))
+    lib(fast).or_render(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
+                        C.c_uint32(samples), C.c_uint32(sample_base), C.c_uint32(threads), fp(out))
     del keep
     return [out[0], out[1], out[2], out[3]]
 

@@ -330,3 +330,13 @@ def test_homogeneous_cube_alpha_is_beer_lambert(pkg, oracle):
     assert np.abs(z).max() < 5.0
     assert abs(z.mean()) < 4.0 / math.sqrt(len(z))
     assert (cnt > 0).mean() > 0.3
+
+
+def test_fast_oracle_build_computes_the_same_numbers(pkg, oracle):
+    """The -O3 x86-64-v3 build of oracle.c (bench.py's second CPU-baseline
+    build) renders exactly what the -O2 build renders (no contraction)."""
+    wl = pkg.configs.tiny(w=24, h=16, spp=8, scattevs=4)
+    a = oracle.render(wl, seed=3)
+    b = oracle.render(wl, seed=3, fast=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

@@ -28,6 +28,9 @@ DRK_ENV_NULL = 1
 DRK_ENV_MAP = 2
 DRK_MAX_LIGHTS = 8
 DRK_KERNEL_DEPTHCOMP = 0x1
+DRK_WALK_AUTO = 0
+DRK_WALK_OCTREE = 1
+DRK_WALK_GRID = 2
 DRK_ADJ_SEGMENT = 0
 DRK_ADJ_NULL_COLLISION = 1
 
@@ -111,6 +114,8 @@ _SIGS = {
     "drk_kernel_render_weighted_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P]),
     "drk_kernel_set_adjoint_estimator": (C.c_int, [_P, u32]),
     "drk_scene_set_reference_fog": (C.c_int, [_P, _P]),
+    "drk_scene_set_walk": (C.c_int, [_P, u32]),
+    "drk_scene_walk_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     "drk_kernel_count": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, C.POINTER(C.c_ulonglong)]),
     "drk_kernel_adjoint": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _F, _F, _F]),
     "drk_kernel_adjoint_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P]),

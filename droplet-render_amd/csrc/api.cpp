@@ -74,6 +74,12 @@ struct drk_scene {
     float *d_fog = nullptr;
     float *d_sdf = nullptr;
     float *d_ref = nullptr;         // reference FOG bricks of weighted tracking (or null)
+    // Leaf grid (every leaf at depth grid_levels): cell -> leaf node, and
+    // whether the kernels walk it (DRK_WALK_*).
+    uint32_t *d_cells = nullptr;
+    uint32_t grid_levels = 0;
+    float grid_occupancy = 0.0f;
+    uint32_t walk = DRK_WALK_AUTO;
     uint32_t node_count = 0;
     uint32_t leafx[2] = {0, 0};
     uint32_t lvoxc = 0;
@@ -215,8 +221,50 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
             }
         }
     }
+    // Leaf grid: when every leaf reachable from the root sits at one depth L
+    // (as the reference's build makes it), the cell of each leaf follows
+    // from the child slots on its path (slot bit 0: x, 1: y, 2: z).
+    std::vector<uint32_t> cells;
+    uint32_t grid_levels = 0;
+    float grid_occ = 0.0f;
+    {
+        struct Item { uint32_t node, level, x, y, z; };
+        std::vector<Item> todo{{0u, 0u, 0u, 0u, 0u}};
+        std::vector<Item> leaves;
+        int depth = -1;
+        bool ok = !(ob[0].volx[0] != DRK_NO_BRICK || ob[0].volx[1] != DRK_NO_BRICK);
+        size_t visits = 0;
+        while (ok && !todo.empty()) {
+            const Item it = todo.back();
+            todo.pop_back();
+            if (++visits > (8u << 20)) { ok = false; break; }
+            const drk_octree_node &n = ob[it.node];
+            if (n.volx[0] != DRK_NO_BRICK || n.volx[1] != DRK_NO_BRICK) {
+                if (depth < 0) depth = (int)it.level;
+                ok = depth == (int)it.level;
+                leaves.push_back(it);
+                continue;
+            }
+            if (it.level >= kMaxGridLevels) { ok = false; break; }
+            for (uint32_t c = 0; c < 8; ++c)
+                if (n.chn[c]) todo.push_back({n.chn[c], it.level + 1, 2 * it.x + (c & 1), 2 * it.y + ((c >> 1) & 1),
+                                              2 * it.z + (c >> 2)});
+        }
+        if (ok && depth >= 1) {
+            grid_levels = (uint32_t)depth;
+            const size_t g = (size_t)1 << grid_levels;
+            cells.assign(g * g * g, 0u);
+            for (const Item &it : leaves) cells[(it.z * g + it.y) * g + it.x] = it.node;
+            size_t occ = 0;
+            for (uint32_t v : cells) occ += v != 0;
+            grid_occ = (float)occ / (float)cells.size();
+        }
+    }
     drk_scene *s = new (std::nothrow) drk_scene;
     if (!s) return set_error(DRK_ENOMEM, "drk_scene_create: host allocation");
+    s->grid_levels = grid_levels;
+    s->grid_occupancy = grid_occ;
+    if (const char *w = std::getenv("DRK_WALK")) s->walk = (uint32_t)std::atoi(w);
     s->node_count = node_count;
     s->h_ob.assign(ob, ob + node_count);
     s->leafx[0] = leafx_sdf;
@@ -257,6 +305,10 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
         }
         e = dev_malloc(&s->d_bce, bce.size() * sizeof(float));
         if (e == hipSuccess) e = hipMemcpy(s->d_bce, bce.data(), bce.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !cells.empty()) {
+        e = dev_malloc(&s->d_cells, cells.size() * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemcpy(s->d_cells, cells.data(), cells.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     }
     size_t nb = (size_t)leafx_fog * lvoxc * lvoxc * lvoxc;
     if (e == hipSuccess && nb) e = dev_malloc(&s->d_fog, nb * sizeof(float));
@@ -302,7 +354,38 @@ int drk_scene_set_reference_fog(drk_scene *s, const float *pvol_ref) {
     return DRK_OK;
 }
 
+// Whether launches on this scene walk the leaf grid: forced by
+// drk_scene_set_walk, or (auto) when the grid exists and at least
+// kGridMinOccupancy of its cells hold a leaf (in sparser grids most steps
+// would cross empty cells that the octree walk skips a subtree at a time).
+static bool scene_uses_grid(const drk_scene *s) {
+    if (!s->d_cells || s->walk == DRK_WALK_OCTREE) return false;
+    return s->walk == DRK_WALK_GRID || s->grid_occupancy >= kGridMinOccupancy;
+}
+static void set_walk_params(const drk_scene *s, RenderParams &P) {
+    const bool g = scene_uses_grid(s);
+    P.cells = g ? s->d_cells : nullptr;
+    P.grid_levels = g ? s->grid_levels : 0u;
+}
+
+int drk_scene_set_walk(drk_scene *s, uint32_t walk) {
+    if (!s) return set_error(DRK_EINVAL, "drk_scene_set_walk: NULL scene");
+    if (walk > DRK_WALK_GRID) return set_error(DRK_EINVAL, "drk_scene_set_walk: unknown walk %u", walk);
+    s->walk = walk;
+    return DRK_OK;
+}
+
+int drk_scene_walk_info(const drk_scene *s, uint32_t *grid_levels, float *occupancy, int *uses_grid) {
+    if (!s) return set_error(DRK_EINVAL, "drk_scene_walk_info: NULL scene");
+    if (grid_levels) *grid_levels = s->d_cells ? s->grid_levels : 0u;
+    if (occupancy) *occupancy = s->grid_occupancy;
+    if (uses_grid) *uses_grid = scene_uses_grid(s) ? 1 : 0;
+    return DRK_OK;
+}
+
 const drk_octree_node *drk_internal_scene_nodes(const drk_scene *s) { return s ? s->d_ob : nullptr; }
+const uint32_t *drk_internal_scene_cells(const drk_scene *s) { return s && scene_uses_grid(s) ? s->d_cells : nullptr; }
+uint32_t drk_internal_scene_grid_levels(const drk_scene *s) { return s && scene_uses_grid(s) ? s->grid_levels : 0u; }
 const uint32_t *drk_internal_scene_links(const drk_scene *s) { return s ? s->d_chx : nullptr; }
 
 void drk_scene_destroy(drk_scene *s) {
@@ -313,6 +396,7 @@ void drk_scene_destroy(drk_scene *s) {
     if (s->d_fog) (void)hipFree(s->d_fog);
     if (s->d_sdf) (void)hipFree(s->d_sdf);
     if (s->d_ref) (void)hipFree(s->d_ref);
+    if (s->d_cells) (void)hipFree(s->d_cells);
     if (s->d_ev) (void)hipFree(s->d_ev);
     if (s->d_evbins) (void)hipFree(s->d_evbins);
     delete s;
@@ -516,6 +600,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         return set_error(DRK_EINVAL, "render: tile origin overflows");
     if ((uint64_t)sample_base + samples > 0xffffffffull) return set_error(DRK_EINVAL, "render: sample counter overflows");
     RenderParams P = k->P;
+    set_walk_params(k->scene, P);
     P.x0 = x0;
     P.y0 = y0;
     P.rx = tilex & ~1u;   // K_ParallelRender iterates rx/2 packets of 2 (src/kernel.cpp:453)
@@ -932,6 +1017,7 @@ int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
              (s->d_bce ? (size_t)s->leafx[1] * 4 * sizeof(float) : 0);
     if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
     if (s->d_ref) *bytes += (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
+    if (s->d_cells) *bytes += ((size_t)1 << (3 * s->grid_levels)) * sizeof(uint32_t);
     return DRK_OK;
 }
 
@@ -1038,6 +1124,7 @@ static int batch_plan(drk_batch *b) {
         RenderParams &P = b->hP[j];
         WorkParams &W = b->hW[j];
         P = k->P;
+        set_walk_params(k->scene, P);
         uint32_t rx, ry;
         rendered_extent(k, J.x0, J.y0, J.tilex, J.tiley, rx, ry);
         P.x0 = J.x0;

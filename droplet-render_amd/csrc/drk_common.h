@@ -109,6 +109,8 @@ constexpr uint32_t kChildLeaf = 0x80000000u;
 struct RenderParams {
     const drk_octree_node *ob;
     const uint32_t *chx;      // node_count x 8 child links (index | kChildLeaf)
+    const uint32_t *cells;    // leaf grid: (2^grid_levels)^3 leaf node indices (0: empty), x fastest; or null
+    uint32_t grid_levels;     // depth of every leaf (0: walk the octree)
     const float *pvol_fog;
     const float *pvol_sdf;
     const float *pvol_ref;    // weighted tracking: the reference FOG bricks that decide collisions

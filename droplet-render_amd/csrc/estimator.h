@@ -247,6 +247,170 @@ __device__ __forceinline__ int trav_next(const uint32_t *__restrict__ chx, Trave
 }
 
 // ---------------------------------------------------------------------------
+// Leaf-grid walk. The reference's build puts every leaf at the same depth L
+// (Octree::BuildPath, src/scene.cpp:273), so the leaves are the occupied
+// cells of a 2^L-per-axis grid over the root cube. The t-interval of a cell
+// along one axis is a pair of boundaries of the binary subdivision of the
+// root's [t0, t1] on that axis, B[k], each the midpoint 0.5f*(lo+hi) the
+// Revelles walk computes at the level where k is first a midpoint; so a
+// cell's [max entries, min exits] is bit-identical to the interval the
+// Revelles walk gives its leaf. Stepping from cell to cell across the exit
+// plane chosen with next_node's comparisons visits the leaves in the
+// Revelles order, with one step per cell instead of a climb and descent
+// through the tree per leaf (empty cells are stepped over). Two situations
+// are handed to the Revelles walk, so that the sequence stays exact:
+// non-finite root intervals (a zero direction component) or a start point
+// on a cell boundary (ties in the descent), and a step where the new entry
+// time equals another axis's entry time (the Revelles descent into the next
+// subtree may then pick a lower cell). The hand-over rebuilds the Revelles
+// stack for the current cell from the root (grid_to_stack).
+//
+// State while walking the grid: T.lvl = kTravGrid, T.t0/T.t1 = the cell's
+// entry and exit times per axis (mirrored frame), stk[0] = the cell index
+// per axis (8 bits each) and bit 31 set while the start cell is unvisited.
+// ---------------------------------------------------------------------------
+constexpr int kTravGrid = -3;
+constexpr uint32_t kGridPending = 0x80000000u;
+
+// B[k] of the subdivision of [t0, t1] into 2^L cells (0 < k < 2^L).
+__device__ __forceinline__ float grid_boundary(float t0, float t1, uint32_t L, uint32_t k) {
+#pragma clang fp contract(off)
+    float lo = t0, hi = t1, res = t1;
+    uint32_t base = 0;
+    for (uint32_t span = (1u << L) >> 1; span; span >>= 1) {
+        const float mid = 0.5f * (lo + hi);
+        const uint32_t mi = base + span;
+        res = k == mi ? mid : res;
+        const bool up = k >= mi;
+        lo = up ? mid : lo;
+        hi = up ? hi : mid;
+        base = up ? mi : base;
+    }
+    return k == 0 ? t0 : res;
+}
+
+// Start of a ray's grid walk: the cell the Revelles walk reaches first (the
+// one holding the root entry point, or the ray origin when that lies inside
+// the root). False when the walk must be the Revelles one (see above).
+__device__ __forceinline__ bool grid_start(F3 t0, F3 t1, uint32_t L, Traversal &T, uint32_t *stk) {
+#pragma clang fp contract(off)
+    if (!(isfinite(t0.x) && isfinite(t0.y) && isfinite(t0.z) && isfinite(t1.x) && isfinite(t1.y) && isfinite(t1.z)))
+        return false;
+    const float m = fmaxf(fmaxf(t0.x, t0.y), t0.z);
+    const float s = fmaxf(m, 0.0f);
+    const float a0[3] = {t0.x, t0.y, t0.z}, a1[3] = {t1.x, t1.y, t1.z};
+    float E[3], X[3];
+    uint32_t packed = kGridPending;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lo = a0[k], hi = a1[k];
+        uint32_t id = 0;
+        for (uint32_t span = (1u << L) >> 1; span; span >>= 1) {
+            const float mid = 0.5f * (lo + hi);
+            const bool up = mid < s;
+            lo = up ? mid : lo;
+            hi = up ? hi : mid;
+            id += up ? span : 0u;
+        }
+        E[k] = lo;
+        X[k] = hi;
+        packed |= id << (8 * k);
+        // the start point on a boundary of this axis (the root entry plane excepted)
+        ok = ok && !(X[k] == s) && !(E[k] == s && !(m >= 0.0f && a0[k] == m));
+    }
+    if (!ok) return false;
+    T.t0 = mk3(E[0], E[1], E[2]);
+    T.t1 = mk3(X[0], X[1], X[2]);
+    T.lvl = kTravGrid;
+    stk[0] = packed;
+    return true;
+}
+
+// Rebuilds the Revelles stack for the walk standing at grid cell idx (the
+// state after visiting it, or after visiting the missing subtree that
+// contains it), so that trav_next continues from there.
+__device__ __forceinline__ void grid_to_stack(const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk,
+                                              const float *root, uint32_t stride, uint32_t L, uint32_t packed) {
+    F3 t0, t1;
+    trav_root(root, stride, t0, t1);
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        const uint32_t sh = L - 1 - l;
+        const uint32_t c = (((packed >> sh) & 1u) << 2) | (((packed >> (8 + sh)) & 1u) << 1) | ((packed >> (16 + sh)) & 1u);
+        F3 ct0, ct1;
+        child_interval(t0, t1, c, ct0, ct1);
+        const uint32_t nxt = (c == 7) ? 8u : next_node(ct1, c);
+        stk[l * stride] = node | (nxt << 24) | (c << 28);
+        T.lvl = (int)l;
+        T.t0 = t0;
+        T.t1 = t1;
+        const uint32_t link = chx[node * 8u + build_slot(c ^ T.a)];
+        const uint32_t ch = link & kStkNodeMask;
+        if (ch == 0 || l + 1 == L) return;
+        node = ch;
+        t0 = ct0;
+        t1 = ct1;
+    }
+}
+
+// One call of the grid walk, with the contract of trav_next: at most
+// `budget` cell steps, then kTravPending; a hand-over continues in
+// trav_next within the same budget.
+__device__ __forceinline__ int grid_next(const uint32_t *__restrict__ cells, uint32_t L,
+                                         const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk,
+                                         const float *root, uint32_t stride, uint32_t &leaf, float &tra, float &trb,
+                                         uint32_t &touches, uint32_t budget) {
+    const uint32_t n = 1u << L;
+    uint32_t packed = stk[0];
+    for (uint32_t steps = 0;; ++steps) {
+        if (steps >= budget) {
+            stk[0] = packed;
+            return kTravPending;
+        }
+        if (packed & kGridPending) {
+            packed &= ~kGridPending;
+        } else {
+            // next_node's choice of the exit plane (ties as the reference breaks them)
+            const bool lxy = T.t1.x < T.t1.y, lxz = T.t1.x < T.t1.z, lyz = T.t1.y < T.t1.z;
+            const uint32_t ax = lxy ? (lxz ? 0u : 2u) : (lyz ? 1u : 2u);
+            const float m = ax == 0 ? T.t1.x : (ax == 1 ? T.t1.y : T.t1.z);
+            const bool tie = (ax != 0 && T.t0.x >= m) || (ax != 1 && T.t0.y >= m) || (ax != 2 && T.t0.z >= m);
+            if (tie) {
+                grid_to_stack(chx, T, stk, root, stride, L, packed);
+                return trav_next(chx, T, stk, root, stride, leaf, tra, trb, touches, budget - steps);
+            }
+            const uint32_t id = (packed >> (8 * ax)) & 0xffu;
+            if (id + 1 == n) {
+                T.lvl = -1;
+                return kTravExhausted;
+            }
+            packed += 1u << (8 * ax);
+            F3 r0, r1;
+            trav_root(root, stride, r0, r1);
+            const float lo = ax == 0 ? r0.x : (ax == 1 ? r0.y : r0.z);
+            const float hi = ax == 0 ? r1.x : (ax == 1 ? r1.y : r1.z);
+            const float nx = grid_boundary(lo, hi, L, id + 2);
+            if (ax == 0) { T.t0.x = m; T.t1.x = nx; }
+            else if (ax == 1) { T.t0.y = m; T.t1.y = nx; }
+            else { T.t0.z = m; T.t1.z = nx; }
+        }
+        // the cell's leaf, if any (grid indices are in the unmirrored frame)
+        const uint32_t ix = packed & 0xffu, iy = (packed >> 8) & 0xffu, iz = (packed >> 16) & 0xffu;
+        const uint32_t ux = (T.a & 4) ? n - 1 - ix : ix, uy = (T.a & 2) ? n - 1 - iy : iy, uz = (T.a & 1) ? n - 1 - iz : iz;
+        const uint32_t cell = cells[(uz * n + uy) * n + ux];
+        touches += 1;
+        if (cell) {
+            leaf = cell;
+            tra = fmaxf(fmaxf(T.t0.x, T.t0.y), T.t0.z);
+            trb = fminf(fminf(T.t1.x, T.t1.y), T.t1.z);
+            stk[0] = packed;
+            return kTravLeaf;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // SampleVoxelSpace (src/kernel.cpp:96-126): trilinear lookup in one
 // lvoxc^3 vertex-centred brick, lerping x, then y, then z; lerp(a,b,t) is
 // t*(b-a)+a with separate rounding, as the reference's SSE3 build computes it.

@@ -130,6 +130,7 @@ __global__ void k_trilinear(const float *brick, uint32_t lvoxc, float4 ce, const
 }
 
 __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node *ob, const uint32_t *chx,
+                                                          const uint32_t *cells, uint32_t grid_levels,
                                                           const float *ro, const float *rd,
                                                           uint32_t n, uint32_t max_leaves, uint32_t *nodes,
                                                           float *t0o, float *t1o, uint32_t *count) {
@@ -156,16 +157,20 @@ __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node 
             }
             cnt = 1;
         } else {
-            F3 tm;
-            {
+            if (!(grid_levels && grid_start(t0, t1, grid_levels, T, stk))) {
+                F3 tm;
+                {
 #pragma clang fp contract(off)
-                tm = mk3(0.5f * (t0.x + t1.x), 0.5f * (t0.y + t1.y), 0.5f * (t0.z + t1.z));
+                    tm = mk3(0.5f * (t0.x + t1.x), 0.5f * (t0.y + t1.y), 0.5f * (t0.z + t1.z));
+                }
+                stk[0] = first_node(t0, tm) << 24;
+                T.lvl = 0;
             }
-            stk[0] = first_node(t0, tm) << 24;
-            T.lvl = 0;
             uint32_t leaf;
             float a0, a1;
-            while (trav_next(chx, T, stk, root, kProbeBlock, leaf, a0, a1, touches, ~0u) == kTravLeaf) {
+            while ((T.lvl == kTravGrid
+                        ? grid_next(cells, grid_levels, chx, T, stk, root, kProbeBlock, leaf, a0, a1, touches, ~0u)
+                        : trav_next(chx, T, stk, root, kProbeBlock, leaf, a0, a1, touches, ~0u)) == kTravLeaf) {
                 if (cnt < max_leaves) {
                     nodes[(size_t)i * max_leaves + cnt] = leaf;
                     t0o[(size_t)i * max_leaves + cnt] = a0;
@@ -210,6 +215,8 @@ using namespace drk;
 // The scene's device octree lives in the opaque handle; api.cpp exposes it here.
 extern "C" const drk_octree_node *drk_internal_scene_nodes(const drk_scene *s);
 extern "C" const uint32_t *drk_internal_scene_links(const drk_scene *s);
+extern "C" const uint32_t *drk_internal_scene_cells(const drk_scene *s);
+extern "C" uint32_t drk_internal_scene_grid_levels(const drk_scene *s);
 
 extern "C" {
 
@@ -403,7 +410,15 @@ int drk_probe_traverse(const drk_scene *scene, const float *ro, const float *rd,
     PTRY(B.alloc(dt0, (size_t)n * max_leaves), "probe alloc");
     PTRY(B.alloc(dt1, (size_t)n * max_leaves), "probe alloc");
     PTRY(B.alloc(dc, n), "probe alloc");
-    k_traverse<<<blocks(n), kProbeBlock>>>(drk_internal_scene_nodes(scene), drk_internal_scene_links(scene), dro, drd, n, max_leaves, dn, dt0, dt1, dc);
+    // entries past a ray's leaf count stay zero, as the oracle leaves them
+    if (max_leaves) {
+        PTRY(hipMemset(dn, 0, 4 * (size_t)n * max_leaves), "probe clear");
+        PTRY(hipMemset(dt0, 0, 4 * (size_t)n * max_leaves), "probe clear");
+        PTRY(hipMemset(dt1, 0, 4 * (size_t)n * max_leaves), "probe clear");
+    }
+    k_traverse<<<blocks(n), kProbeBlock>>>(drk_internal_scene_nodes(scene), drk_internal_scene_links(scene),
+                                           drk_internal_scene_cells(scene), drk_internal_scene_grid_levels(scene), dro,
+                                           drd, n, max_leaves, dn, dt0, dt1, dc);
     PTRY(hipGetLastError(), "probe launch");
     if (max_leaves) {
         PTRY(hipMemcpy(nodes, dn, 4 * (size_t)n * max_leaves, hipMemcpyDeviceToHost), "probe download");

@@ -233,6 +233,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         if (kCount) cnt_nodes += 1;
         if (is_leaf(P.ob[0])) {
             T.lvl = -2;    // root leaf pending
+        } else if (P.grid_levels && grid_start(t0, t1, P.grid_levels, T, stk)) {
+            // walking the leaf grid from the start cell
         } else {
             F3 tm = mk3(0.5f * (t0.x + t1.x), 0.5f * (t0.y + t1.y), 0.5f * (t0.z + t1.z));
             stk[0] = 0u | (first_node(t0, tm) << 24);
@@ -460,7 +462,10 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 T.lvl = -1;
                 found = true;
             } else {
-                const int r = trav_next(P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
+                const int r = T.lvl == kTravGrid
+                                  ? grid_next(P.cells, P.grid_levels, P.chx, T, stk, root, stride, leaf, tra, trbn,
+                                              cnt_nodes, kTravBudget)
+                                  : trav_next(P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
                 if (r == kTravPending) continue_trav = true;
                 found = r == kTravLeaf;
             }

@@ -42,6 +42,10 @@ inline void plan_chunks(uint32_t samples, uint32_t chunk, bool guided, uint32_t 
     nchunks = nfull;
     for (uint32_t rem = samples - nfull * chunk; rem > 0; rem -= (rem + 1) / 2) ++nchunks;
 }
+// Leaf grid: at most 2^6 cells per axis (1 MiB of cells); walked by
+// default when at least this fraction of its cells hold a leaf.
+constexpr uint32_t kMaxGridLevels = 6;
+constexpr float kGridMinOccupancy = 0.125f;
 constexpr uint32_t kCounterStride = 16;   // u32 words: one 64-byte line per band counter
 // Counter block of a launch: kMaxBands band counters, then one line whose
 // first word is the launch's status (1: a wave hit the round limit). The

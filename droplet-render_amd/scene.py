@@ -125,6 +125,16 @@ class Scene:
         if q is not None:
             self.ob["qval"][:, L.DRK_VOLUME_FOG] = q
 
+    def set_walk(self, walk):
+        """DRK_WALK_AUTO, DRK_WALK_OCTREE or DRK_WALK_GRID (next launches)."""
+        L.check(L.lib().drk_scene_set_walk(self.device(), int(walk)))
+
+    def walk_info(self):
+        """(grid depth or 0, fraction of occupied grid cells, launches walk the grid)."""
+        lv, occ, g = C.c_uint32(), C.c_float(), C.c_int()
+        L.check(L.lib().drk_scene_walk_info(self.device(), C.byref(lv), C.byref(occ), C.byref(g)))
+        return int(lv.value), float(occ.value), bool(g.value)
+
     def set_reference_fog(self, pvol_ref):
         """Reference FOG bricks for weighted tracking (host array or torch
         tensor of leafx*lvoxc^3 floats); None drops them."""

@@ -92,6 +92,23 @@ int  drk_scene_create(const drk_octree_node *ob, uint32_t node_count,
                       const float *pvol_sdf, uint32_t leafx_sdf,
                       const float *pvol_fog, uint32_t leafx_fog,
                       uint32_t lvoxc, drk_scene **out);
+/*
+ * Octree walk of the kernels (no reference counterpart; the leaf sequence
+ * and intervals are the Revelles traversal's, src/KernelOctree.cpp:9-349,
+ * either way). When every leaf sits at one depth L <= 6 (the reference's
+ * build), drk_scene_create also builds the leaf grid (2^L cells per axis ->
+ * leaf node) and the kernels can step through it cell by cell instead of
+ * climbing the tree; DRK_WALK_AUTO (the default) walks the grid when at
+ * least 1/8 of its cells hold a leaf. drk_scene_walk_info reports the
+ * grid's depth (0: none), the fraction of occupied cells and whether
+ * launches walk it. Takes effect at the next launch on the scene.
+ */
+#define DRK_WALK_AUTO   0
+#define DRK_WALK_OCTREE 1
+#define DRK_WALK_GRID   2
+int drk_scene_set_walk(drk_scene *scene, uint32_t walk);
+int drk_scene_walk_info(const drk_scene *scene, uint32_t *grid_levels, float *occupancy, int *uses_grid);
+
 /* Replaces the FOG bricks and per-leaf majorants in place (same leaf count);
  * used between optimisation steps. qval may be NULL to keep the majorants. */
 int  drk_scene_update_fog(drk_scene *scene, const float *pvol_fog, const float *qval_fog);

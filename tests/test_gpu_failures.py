@@ -121,7 +121,8 @@ def test_allocation_failures_are_clean(pkg, gpu):
             L.set_alloc_gate(None)
     assert fell_back >= 1
     torch.cuda.synchronize()
-    assert abs(torch.cuda.mem_get_info()[0] - free0) < (4 << 20)
+    # nothing left allocated (the runtime hands memory out in pages of a few MiB)
+    assert torch.cuda.mem_get_info()[0] >= free0 - (16 << 20)
     img, g = _workflow(pkg, wl)
     for a, b in zip(img, img0):
         assert np.array_equal(a, b)

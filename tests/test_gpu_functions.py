@@ -133,28 +133,37 @@ def test_trilinear_bit_exact(pkg, oracle, gpu):
         assert np.array_equal(pkg.probe.trilinear(brick, lvoxc, ce, p), oracle.voxel(brick, p, ce, lvoxc))
 
 
-@pytest.mark.parametrize("cfg", ["c0", "spheres64"])
-def test_traversal_bit_exact(pkg, oracle, gpu, cfg):
+@pytest.mark.parametrize("walk", ["octree", "grid"])
+@pytest.mark.parametrize("cfg", ["c0", "spheres64", "c1"])
+def test_traversal_bit_exact(pkg, oracle, gpu, cfg, walk):
     """Leaf sequences (node indices and [tr0, tr1]) of camera rays and of
     secondary rays starting inside the volume, against the oracle's recursive
-    full traverser."""
+    full traverser, for both walks of the kernels (the Revelles stack walk
+    and the leaf-grid walk with its hand-overs)."""
     if cfg == "c0":
         wl = pkg.configs.c0(w=64, h=64)
         sc = wl.scene
         ys, xs = np.mgrid[0:64, 0:64]
         ro, rd, _ = oracle.camera_rays(wl.viewi, wl.proji, 64, 64, xs.ravel(), ys.ravel())
     else:
-        sc = pkg.Scene.from_grid(pkg.configs.synth_spheres(64, 3, spheres=12))
+        sc = (pkg.Scene.from_grid(pkg.configs.synth_spheres(64, 3, spheres=12)) if cfg == "spheres64"
+              else pkg.configs.c1(w=8, h=8).scene)
         ro = np.zeros((0, 3), np.float32)
         rd = np.zeros((0, 3), np.float32)
+    sc.set_walk(pkg._lib.DRK_WALK_GRID if walk == "grid" else pkg._lib.DRK_WALK_OCTREE)
+    assert sc.walk_info()[2] == (walk == "grid")
     rng = np.random.default_rng(8)
-    m = 6000
+    m = 20000
     ro2 = rng.uniform(0.0, 1.0, (m, 3)).astype(np.float32)
     rd2 = rng.normal(size=(m, 3))
     rd2 /= np.linalg.norm(rd2, axis=1, keepdims=True)
-    # axis-aligned and diagonal directions exercise the tie rules
+    # axis-aligned and diagonal directions exercise the tie rules; origins on
+    # the cell boundaries exercise the hand-over to the Revelles walk
     rd2[:200] = np.array([[1, 0, 0], [0, -1, 0], [0, 0, 1], [-1, 0, 0]] * 50)
     rd2[200:300] = np.array([1, 1, 1]) / math.sqrt(3)
+    rd2[300:600] = rng.choice([-1.0, 1.0], (300, 3)) / math.sqrt(3)
+    c = sc.ob["ce"][0]
+    ro2[300:900] = (c[:3] + rng.integers(-16, 17, (600, 3)) / 16.0 * c[3]).astype(np.float32)
     ro = np.concatenate([ro, ro2]).astype(np.float32)
     rd = np.concatenate([rd, rd2]).astype(np.float32)
     g = pkg.probe.traverse(sc, ro, rd)

@@ -323,3 +323,20 @@ def test_homogeneous_cube_alpha_is_beer_lambert_on_gpu(pkg, oracle, gpu):
     assert np.abs(z).max() < 5.0, np.abs(z).max()
     assert abs(z.mean()) < 4.0 / math.sqrt(len(z))
     assert (cnt > 0).mean() > 0.3
+
+
+@pytest.mark.parametrize("cfg", ["c0", "tiny"])
+def test_grid_and_octree_walks_render_the_same(pkg, gpu, cfg):
+    """The leaf-grid walk visits the same leaves with the same intervals as
+    the Revelles walk, so renders are bit-identical (same keys)."""
+    wl = pkg.configs.c0(w=64, h=48) if cfg == "c0" else pkg.configs.tiny(w=48, h=40, spp=16)
+    lv, occ, _ = wl.scene.walk_info()
+    assert lv >= 1 and occ > 0
+    out = {}
+    for walk in (pkg._lib.DRK_WALK_OCTREE, pkg._lib.DRK_WALK_GRID):
+        wl.scene.set_walk(walk)
+        out[walk] = render_gpu(wl, 33)
+    wl.scene.set_walk(pkg._lib.DRK_WALK_AUTO)
+    for a, b in zip(out[pkg._lib.DRK_WALK_OCTREE], out[pkg._lib.DRK_WALK_GRID]):
+        assert np.array_equal(a, b)
+    assert out[pkg._lib.DRK_WALK_GRID][0][..., 3].sum() > 0

@@ -386,10 +386,8 @@ __device__ __forceinline__ int grid_next(const uint32_t *__restrict__ cells, uin
                 return kTravExhausted;
             }
             packed += 1u << (8 * ax);
-            F3 r0, r1;
-            trav_root(root, stride, r0, r1);
-            const float lo = ax == 0 ? r0.x : (ax == 1 ? r0.y : r0.z);
-            const float hi = ax == 0 ? r1.x : (ax == 1 ? r1.y : r1.z);
+            // the root interval of that axis (see trav_set_root's layout)
+            const float lo = root[ax * stride], hi = root[(3 + ax) * stride];
             const float nx = grid_boundary(lo, hi, L, id + 2);
             if (ax == 0) { T.t0.x = m; T.t1.x = nx; }
             else if (ax == 1) { T.t0.y = m; T.t1.y = nx; }

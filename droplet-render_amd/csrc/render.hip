@@ -60,6 +60,12 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 #ifndef DRK_TRAV_BUDGET
 #define DRK_TRAV_BUDGET 4
 #endif
+// Cell steps of the leaf-grid walk per round: one (a step costs far less
+// than a Revelles node visit, and a longer loop only holds the wave's other
+// lanes; C1 with budgets 1/2/3/4: 29.5/31.2/32.8/33.6 ms).
+#ifndef DRK_GRID_BUDGET
+#define DRK_GRID_BUDGET 1
+#endif
 #ifndef DRK_UNIT_RESERVE
 #define DRK_UNIT_RESERVE 32
 #endif
@@ -67,6 +73,7 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 #define DRK_END_BATCH 32
 #endif
 constexpr uint32_t kTravBudget = DRK_TRAV_BUDGET;
+constexpr uint32_t kGridBudget = DRK_GRID_BUDGET;
 constexpr uint32_t kEndBatch = DRK_END_BATCH;
 enum : uint32_t { RK_PRIMARY = 0, RK_CONT = 1, RK_SHADOW = 2, RK_SHADOW_PASS = 3 };
 
@@ -464,7 +471,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             } else {
                 const int r = T.lvl == kTravGrid
                                   ? grid_next(P.cells, P.grid_levels, P.chx, T, stk, root, stride, leaf, tra, trbn,
-                                              cnt_nodes, kTravBudget)
+                                              cnt_nodes, kGridBudget)
                                   : trav_next(P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
                 if (r == kTravPending) continue_trav = true;
                 found = r == kTravLeaf;

@@ -123,6 +123,7 @@ struct drk_kernel {
     uint32_t *status = nullptr;     // counter block of the last launch (its status word at kStatusWord)
     char *scratch = nullptr;        // this device's fixed scratch (allocation-free render), or null
     uint32_t chunk_override = 0;
+    uint32_t max_chunks = (uint32_t)kMaxAutoChunks;   // default chunking: at most this many per pixel
     uint32_t grid_override = 0;
     bool depthcomp = false;
     // Live kernel timing: event pairs recorded around each k_render launch.
@@ -556,6 +557,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     }
     k->status = k->d_counter;
     if (const char *c = std::getenv("DRK_CHUNK")) k->chunk_override = (uint32_t)std::atoi(c);
+    if (const char *c = std::getenv("DRK_MAX_CHUNKS")) k->max_chunks = std::max(1, std::atoi(c));
     if (const char *g = std::getenv("DRK_GRID")) k->grid_override = (uint32_t)std::atoi(g);
     *out = k;
     return DRK_OK;
@@ -645,8 +647,8 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         // make a refill cheap. Up to kMaxAutoChunks chunks (one sample each
         // for up to that many samples per pixel), fewer where the per-chunk
         // partial sums would exceed kMaxPartialBytes.
-        const uint64_t bytes_per_chunk = (uint64_t)npix * 16 * (sumsq ? 4 : 2);
-        uint64_t nch = std::min<uint64_t>(samples, kMaxAutoChunks);
+        const uint64_t bytes_per_chunk = (uint64_t)npix * 16 * partial_buffers(mode);
+        uint64_t nch = std::min<uint64_t>(samples, k->max_chunks);
         nch = std::min<uint64_t>(nch, std::max<uint64_t>(1, kMaxPartialBytes / std::max<uint64_t>(bytes_per_chunk, 1)));
         nch = std::max<uint64_t>(1, nch);
         chunk = (uint32_t)((samples + nch - 1) / nch);
@@ -761,7 +763,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         }
     }
     if (nchunks > 1 && !adj && noalloc) {
-        const size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
+        const size_t need = (size_t)partial_buffers(mode) * nchunks * npix * 16;
         if (need > kScratchBytes - kScratchPartialOffset)
             return set_error(DRK_EINVAL,
                              "render (no allocation): %u pixels x %u sample chunks need %zu bytes of partial sums; "
@@ -769,7 +771,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
                              npix, nchunks, need, (size_t)(kScratchBytes - kScratchPartialOffset));
         W.partial = reinterpret_cast<float *>(k->scratch + kScratchPartialOffset);
     } else if (nchunks > 1 && !adj) {
-        size_t need = (size_t)(sumsq ? 4 : 2) * nchunks * npix * 16;
+        size_t need = (size_t)partial_buffers(mode) * nchunks * npix * 16;
         if (need > k->partial_bytes) {
             if (k->d_partial) (void)hipFree(k->d_partial);
             k->d_partial = nullptr;

@@ -759,15 +759,17 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     st = ST_PATH;
                 } else {
                     // Unit complete: write its sums.
-                    size_t o = (size_t)chunk_cur * W.npix + pix_local;
-                    float4 *oL = W.partial ? reinterpret_cast<float4 *>(W.partial) : nullptr;
                     if (W.partial) {
-                        size_t plane = (size_t)W.nchunks * W.npix;
-                        oL[o] = make_float4(accL[0], accL[1], accL[2], accA);
-                        oL[plane + o] = make_float4(accE[0], accE[1], accE[2], accA);
+                        // The unit's buffers side by side: one lane writes
+                        // whole 32-byte (64 with squares) spans.
+                        constexpr uint32_t kNb = kSumSq ? 4u : 2u;
+                        float4 *oP = reinterpret_cast<float4 *>(W.partial) +
+                                     ((size_t)chunk_cur * W.npix + pix_local) * kNb;
+                        oP[0] = make_float4(accL[0], accL[1], accL[2], accA);
+                        oP[1] = make_float4(accE[0], accE[1], accE[2], accA);
                         if (kSumSq) {
-                            oL[2 * plane + o] = make_float4(sqL[0], sqL[1], sqL[2], accA);
-                            oL[3 * plane + o] = make_float4(sqE[0], sqE[1], sqE[2], accA);
+                            oP[2] = make_float4(sqL[0], sqL[1], sqL[2], accA);
+                            oP[3] = make_float4(sqE[0], sqE[1], sqE[2], accA);
                         }
                     } else {
                         uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
@@ -847,22 +849,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 // Sums the per-chunk partials of each pixel in chunk order (deterministic).
-__global__ void k_reduce(WorkParams W, uint32_t rx, uint32_t tilex, uint32_t buffers) {
+template <uint32_t kNb>
+__global__ void k_reduce(WorkParams W, uint32_t rx, uint32_t tilex) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W.npix) return;
-    const float4 *part = reinterpret_cast<const float4 *>(W.partial);
-    size_t plane = (size_t)W.nchunks * W.npix;
+    const float4 *part = reinterpret_cast<const float4 *>(W.partial) + (size_t)i * kNb;
+    const size_t cstride = (size_t)W.npix * kNb;
+    float4 s[kNb];
+#pragma unroll
+    for (uint32_t b = 0; b < kNb; ++b) s[b] = make_float4(0, 0, 0, 0);
+    for (uint32_t c = 0; c < W.nchunks; ++c) {
+#pragma unroll
+        for (uint32_t b = 0; b < kNb; ++b) {
+            const float4 v = part[(size_t)c * cstride + b];
+            s[b].x += v.x; s[b].y += v.y; s[b].z += v.z; s[b].w += v.w;
+        }
+    }
     float *outs[4] = {W.out_l, W.out_e, W.sq_l, W.sq_e};
     uint32_t lx = i % rx, ly = i / rx;
     size_t d = (size_t)ly * tilex + lx;
-    for (uint32_t b = 0; b < buffers; ++b) {
-        float4 s = make_float4(0, 0, 0, 0);
-        for (uint32_t c = 0; c < W.nchunks; ++c) {
-            float4 v = part[b * plane + (size_t)c * W.npix + i];
-            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
-        }
-        reinterpret_cast<float4 *>(outs[b])[d] = s;
-    }
+#pragma unroll
+    for (uint32_t b = 0; b < kNb; ++b) reinterpret_cast<float4 *>(outs[b])[d] = s[b];
+}
+
+static void launch_reduce_kernel(const WorkParams &W, uint32_t rx, uint32_t tilex, uint32_t buffers, hipStream_t s) {
+    const uint32_t nb = (W.npix + 255) / 256;
+    if (buffers == 4) k_reduce<4><<<nb, 256, 0, s>>>(W, rx, tilex);
+    else k_reduce<2><<<nb, 256, 0, s>>>(W, rx, tilex);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -919,7 +932,7 @@ hipError_t launch_render_batch(const RenderParams *dP, const WorkParams *dW, uin
 
 hipError_t launch_reduce(const WorkParams &W, uint32_t rx, uint32_t tilex, uint32_t buffers, hipStream_t s) {
     if (!W.partial || W.npix == 0) return hipSuccess;
-    k_reduce<<<(W.npix + 255) / 256, 256, 0, s>>>(W, rx, tilex, buffers);
+    launch_reduce_kernel(W, rx, tilex, buffers, s);
     return hipGetLastError();
 }
 
@@ -940,9 +953,7 @@ hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t gr
     if (e != hipSuccess) return e;
     if (ev1) (void)hipEventRecord(ev1, s);
     if (W.partial && mode != kModeAdjoint && mode != kModeAdjointNull) {
-        uint32_t nb = (W.npix + 255) / 256;
-        bool sq = (mode == kModeForwardSq || mode == kModeCount);
-        k_reduce<<<nb, 256, 0, s>>>(W, P.rx, P.tilex, sq ? 4u : 2u);
+        launch_reduce_kernel(W, P.rx, P.tilex, partial_buffers(mode), s);
         e = hipGetLastError();
     }
     return e;

@@ -62,7 +62,8 @@ struct WorkParams {
     uint32_t chunk;           // samples per unit in the leading chunks
     uint32_t nfull;           // leading chunks of `chunk` samples (see chunk_start)
     uint32_t nchunks;
-    float *partial;           // [buffers][nchunks][npix] float4 when nchunks > 1, else nullptr
+    float *partial;           // [nchunks][npix][buffers] float4 when nchunks > 1, else nullptr: a unit's
+                              // buffers side by side, written by one lane as one 32/64-byte span
     float *out_l, *out_e;     // tile float4 outputs (stride tilex)
     float *sq_l, *sq_e;       // sum-of-squares outputs or nullptr
     unsigned long long *counts;  // [node touches, brick lookups] for the counting pass
@@ -94,6 +95,9 @@ constexpr int kModeAdjoint = 3;     // density gradients by path replay
 constexpr int kModeRecord = 4;      // sums plus the per-sample path record
 constexpr int kModeWeighted = 5;    // sums of weighted delta tracking against reference densities
 constexpr int kModeAdjointNull = 6; // density gradients by path replay, null-collision terms
+// Float4 buffers a render mode sums per pixel: lights and env, plus their
+// sums of squares in the modes that keep them.
+inline uint32_t partial_buffers(int mode) { return (mode == kModeForwardSq || mode == kModeCount) ? 4u : 2u; }
 
 // Bytes of one (pixel, sample) entry of the path record.
 constexpr size_t kRecordBytes = 40;

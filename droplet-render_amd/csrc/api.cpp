@@ -105,6 +105,8 @@ struct drk_kernel {
     float *h_phb = nullptr;         // host copies, same layout
     float *d_partial = nullptr;
     size_t partial_bytes = 0;
+    void *d_mail = nullptr;         // escape-service mailboxes, two float4 per lane of the grid
+    size_t mail_bytes = 0;
     uint32_t *d_counter = nullptr;
     unsigned long long *d_counts = nullptr;
     float *d_env = nullptr;
@@ -781,6 +783,19 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         }
         W.partial = k->d_partial;
     }
+    // Mailboxes of the escape service (forward modes without the path
+    // record; the allocation-free entry computes escape radiance in place).
+    if (!noalloc && (mode == kModeForward || mode == kModeForwardSq || mode == kModeCount || mode == kModeWeighted)) {
+        const size_t need = (size_t)grid * render_block_size() * 32;
+        if (need > k->mail_bytes) {
+            if (k->d_mail) (void)hipFree(k->d_mail);
+            k->d_mail = nullptr;
+            k->mail_bytes = 0;
+            HIP_TRY(dev_malloc(&k->d_mail, need), "render: escape mailboxes");
+            k->mail_bytes = need;
+        }
+        W.esc_mail = static_cast<float4 *>(k->d_mail);
+    }
     if (count) HIP_TRY(hipMemsetAsync(k->d_counts, 0, 16, s), "render: counters");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (k->timing && !count && !noalloc) {
@@ -1004,7 +1019,7 @@ int drk_kernel_shadow(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, u
 
 int drk_kernel_footprint(const drk_kernel *k, size_t *bytes) {
     if (!k || !bytes) return set_error(DRK_EINVAL, "drk_kernel_footprint: NULL argument");
-    size_t b = (size_t)4 * k->tilex * k->tiley * 16 + k->partial_bytes + k->gradx_bytes + sizeof(KernelConsts);
+    size_t b = (size_t)4 * k->tilex * k->tiley * 16 + k->partial_bytes + k->mail_bytes + k->gradx_bytes + sizeof(KernelConsts);
     b += kCounterWords * sizeof(uint32_t) + 64;
     if (k->d_env) b += (size_t)k->C.env_w * k->C.env_h * 16;
     if (k->d_depth) b += (size_t)k->P.w * k->P.h * sizeof(float);
@@ -1085,6 +1100,7 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->stream) (void)hipStreamSynchronize(k->stream);
     if (k->d_out) (void)hipFree(k->d_out);
     if (k->d_partial) (void)hipFree(k->d_partial);
+    if (k->d_mail) (void)hipFree(k->d_mail);
     if (k->d_counter) (void)hipFree(k->d_counter);
     if (k->d_counts) (void)hipFree(k->d_counts);
     if (k->d_env) (void)hipFree(k->d_env);

@@ -6,6 +6,13 @@
 // or sample boundaries; the one deliberate wait gathers lanes whose ray has
 // ended until enough of them run the ray-end code together (kEndBatch),
 // and new paths and ray starts each run as one converged pass per round.
+// Two long ray-end code blocks that only a few lanes of a wave need at a
+// time, the phase-function half of a scattering event and the sun and sky
+// radiance of an escape, are regrouped across the four waves of a
+// workgroup: lanes queue them in LDS (ballot + mbcnt ranks, one LDS atomic
+// per wave) and a wave that finds enough queued runs them with full lanes,
+// while the queuing lanes go on (the scattering lane traces its shadow ray
+// meanwhile; the escaping lane takes its next work unit).
 // A lane that finishes its work unit (one pixel x a
 // chunk of samples, one sample by default) refills from its wave's reserve
 // of units, which one wave-aggregated atomic on a unit counter (ballot +
@@ -46,8 +53,9 @@ constexpr int kBlock = 256;
 
 // Lane states: needs a work unit, needs its next camera path, needs its
 // pending ray started, traversing to the next leaf, tracking inside a leaf,
-// ray finished (escape, collision or scattering event pending), drained.
-enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4, ST_END = 5, ST_DONE = 6 };
+// ray finished (escape, collision or scattering event pending), drained,
+// waiting for a workgroup service (scattering or escape radiance) to return.
+enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4, ST_END = 5, ST_DONE = 6, ST_WAIT = 7 };
 
 // Scheduling of the state machine. A traversal call runs at most
 // kTravBudget node steps before the wave moves on (a lane with a longer walk
@@ -72,6 +80,49 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 #ifndef DRK_END_BATCH
 #define DRK_END_BATCH 32
 #endif
+// Workgroup scattering service (forward modes): at a scattering event a
+// lane samples only the shadow ray's direction (Philox and the sun cone) and
+// starts tracing it at once; the rest of the event (the phase-function
+// sample, both phase values and sun pdfs, the MIS weights) is queued in LDS
+// and computed by whichever wave of the workgroup next finds kScatterBatch
+// jobs queued (or none of its own lanes able to progress), so that this
+// long code block runs with full waves instead of the few lanes of one wave
+// whose rays end in the same round. The results (the continuation and the
+// shadow ray's weight) are needed only at the shadow ray's end, where a lane
+// whose job has not come back yet waits for it.
+// Workgroup escape service (forward modes without the path record): when a
+// path that escaped after scattering ends a work unit whose sums are still
+// zero (every unit of one sample), the lane writes the unit's sums without
+// the escape radiance and moves on; the sun and sky radiance along the
+// escape direction (the longest code of the ray end, needed by a few lanes
+// at a time) is computed by the wave that next finds kEscapeBatch such jobs
+// queued, which adds it to the unit's sums in memory, in the order the lane
+// would have (path radiance first, then the unit's sums).
+#ifndef DRK_SCATTER_SERVICE
+#define DRK_SCATTER_SERVICE 1
+#endif
+#ifndef DRK_ESCAPE_SERVICE
+#define DRK_ESCAPE_SERVICE 1
+#endif
+#ifndef DRK_SCATTER_BATCH
+#define DRK_SCATTER_BATCH 32
+#endif
+#ifndef DRK_ESCAPE_BATCH
+#define DRK_ESCAPE_BATCH 48
+#endif
+constexpr uint32_t kScatterBatch = DRK_SCATTER_BATCH;
+constexpr uint32_t kEscapeBatch = DRK_ESCAPE_BATCH;
+constexpr uint32_t kQueueScatter = 0, kQueueEscape = 1;
+// Per-lane service flags: the lane's scattering job is back; the lane's
+// escape job (in its mailbox in memory) has not been taken yet.
+constexpr uint32_t kFlagScatterDone = 1u, kFlagEscapeBusy = 2u;
+// Bounds of the waits of the queue protocol: a slot or job wait spins at
+// most kSpinLimit times, and a wave whose live lanes all wait gives up after
+// kStallLimit rounds without a returned job (status word 2). Neither is
+// reached unless the protocol is broken; they keep every wave finite.
+constexpr uint32_t kSpinLimit = 1u << 20;
+constexpr uint32_t kStallLimit = 1u << 22;
+constexpr uint16_t kSlotFree = 0xFFFFu;
 constexpr uint32_t kTravBudget = DRK_TRAV_BUDGET;
 constexpr uint32_t kGridBudget = DRK_GRID_BUDGET;
 constexpr uint32_t kEndBatch = DRK_END_BATCH;
@@ -143,17 +194,65 @@ struct RenderLds {
     // event buffer it has reserved (in LDS, so that every lane sees the
     // updates made under divergent control flow).
     unsigned long long ev_cur[kBlock / 64][2];
+    // Workgroup services (scattering, escape): per queue a ring of queued
+    // lanes' workgroup indices (kSlotFree: empty), the next job to serve and
+    // the next free ring position (both counting up); per lane the service
+    // flags. A scattering job's inputs and results are in the lane's cont[]
+    // slots (a lane queues at most one, and reads cont[] only at the end of
+    // the shadow ray it traces meanwhile); an escape job is in the lane's
+    // mailbox in memory (WorkParams::esc_mail).
+    uint32_t q_flag[kBlock];
+    uint32_t q_head[2], q_tail[2];
+    uint16_t q_ring[2][kBlock];
 };
 
 __device__ __forceinline__ void render_lds_init(RenderLds &S) {
     if ((threadIdx.x & 63) == 0) S.ev_cur[threadIdx.x >> 6][0] = S.ev_cur[threadIdx.x >> 6][1] = 0;
+    S.q_flag[threadIdx.x] = 0u;
+    S.q_ring[0][threadIdx.x] = kSlotFree;
+    S.q_ring[1][threadIdx.x] = kSlotFree;
+    if (threadIdx.x < 2) S.q_head[threadIdx.x] = S.q_tail[threadIdx.x] = 0u;
     load_mie(S.mie);
     __syncthreads();
 }
 
+// Scattering event of a path: MIS between the phase function and lights[0]
+// (src/kernel.cpp:387-430). scatter_phase: from the incoming direction, the
+// shadow ray's direction lrd (sampled from the sun cone) and the phase
+// function's two uniforms, the continuation's direction srd and the
+// balance-heuristic weights w1 (continuation) and w2 (shadow ray); the
+// throughputs are beta * w * albedo.
+__device__ __forceinline__ void scatter_phase(const KernelConsts &K, const RenderParams &P, const MieLds &mie, F3 rd,
+                                              F3 lrd, float u1, float u2, F3 &srd, F3 &w1, F3 &w2) {
+    F3 p1, p3;
+    if (P.phase == DRK_PHASE_MIE) {
+        srd = mie_sample(mie, rd, u1, u2);
+        p1 = mie_eval_rgb(mie, dot3(srd, rd));
+        p3 = mie_eval_rgb(mie, dot3(lrd, rd));
+    } else {
+        srd = hg_sample(P.hg_g, rd, u1, u2);
+        float h1 = hg_eval(P.hg_g, dot3(srd, rd)), h3 = hg_eval(P.hg_g, dot3(lrd, rd));
+        p1 = mk3(h1, h1, h1);
+        p3 = mk3(h3, h3, h3);
+    }
+    float pd1 = sun_pdf(K, srd), pd2 = sun_pdf(K, lrd);
+    w1 = mk3(div_approx(p1.x, p1.x + pd1), div_approx(p1.y, p1.y + pd1), div_approx(p1.z, p1.z + pd1));
+    w2 = mk3(div_approx(p3.x, p3.x + pd2), div_approx(p3.y, p3.y + pd2), div_approx(p3.z, p3.z + pd2));
+}
+
+// The event's four uniforms (counter: sample, event; key: frame pixel) and
+// the shadow ray's direction from the last two.
+__device__ __forceinline__ void scatter_uniforms(const KernelConsts &K, const RenderParams &P, uint32_t key0,
+                                                 uint32_t sample, uint32_t event, float &u1, float &u2, F3 &lrd) {
+    U4 r4 = philox4x32_10(U4{sample, event, 0u, kRngScatter}, key0, P.key1);
+    u1 = u01(r4.x);
+    u2 = u01(r4.y);
+    lrd = sun_sample(K, u01(r4.z), u01(r4.w));
+}
+
 // Runs one launch's work (P, W) to completion in the calling wave: its lanes
 // take work units until the units are drained, then the wave returns.
-template <int kMode>
+template <int kMode, bool kQueue>
 __device__ __forceinline__ void render_body(const RenderParams &P, const WorkParams &W, RenderLds &S) {
     constexpr bool kSumSq = (kMode == kModeForwardSq || kMode == kModeCount);
     constexpr bool kCount = (kMode == kModeCount);
@@ -161,6 +260,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     constexpr bool kAdj = (kMode == kModeAdjoint) || kNull;
     constexpr bool kRec = (kMode == kModeRecord);
     constexpr bool kWt = (kMode == kModeWeighted);       // weighted delta tracking
+    // The adjoint needs the shadow ray's weight when the ray starts, so it
+    // samples its scattering events in place.
+    static_assert(!(kQueue && kAdj), "the adjoint does not use the scattering service");
+    // The escape service writes the unit's sums itself, which the path
+    // record (written at the path's end) does not allow.
+    constexpr bool kEscSvc = kQueue && !kRec && DRK_ESCAPE_SERVICE;
+    constexpr bool kScatSvc = kQueue && DRK_SCATTER_SERVICE;
     MieLds &mie = S.mie;
     unsigned long long (*ev_cur)[2] = S.ev_cur;
 
@@ -207,6 +313,11 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     float smax = 1.0f, kmaj = 1.0f, lo = 0.0f, s0 = 0.0f, tr0 = 0.0f, tr1 = 0.0f, trb = 0.0f, sr = 0.0f, td = 0.0f;
     bool first_step = true;
     uint32_t cnt_nodes = 0, cnt_lookups = 0;
+    // Scattering service: the lane waited for its job at a shadow ray's end
+    // and the job is back (the ray end runs at once); rounds in which every
+    // live lane of the wave waited (wave-uniform).
+    bool svc = false;
+    uint32_t stall_rounds = 0;
     // Adjoint state.
     uint32_t adj_d = 0;   // the pixel's offset in the image-gradient tiles
     float ltot = 0.0f, collected = 0.0f, wseg = 0.0f, seg_a = 0.0f;
@@ -369,6 +480,141 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             }
         }
     };
+    // Workgroup services. A lane queues itself (its workgroup index) after
+    // writing its job: one LDS atomic per wave reserves ring positions, and
+    // each lane waits until its position's previous entry has been taken
+    // before it writes its index.
+    auto queue_push = [&](uint32_t q) {
+        const uint64_t m = __ballot(true);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+        uint32_t base = 0;
+        if (lane_id() == leader)
+            base = __hip_atomic_fetch_add(&S.q_tail[q], cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        base = __shfl(base, (int)leader);
+        const uint32_t idx = (base + rank) & (kBlock - 1);
+        for (uint32_t it = 0;
+             __hip_atomic_load(&S.q_ring[q][idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != kSlotFree; ++it) {
+            if (it >= kSpinLimit) {
+                W.counter[kStatusWord] = 2u;
+                break;
+            }
+        }
+        // Release: the job (in LDS or in memory) is written before the entry
+        // is seen.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __hip_atomic_store(&S.q_ring[q][idx], (uint16_t)threadIdx.x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    // The service for lane e: in its cont[0..2] the incoming direction,
+    // cont[3..5] the shadow ray's direction, cont[6..7] the phase function's
+    // uniforms; out in cont[0..8]: the continuation's direction and the MIS
+    // weights w1 and w2 (the lane applies its throughput and the albedo).
+    auto scatter_service = [&](uint32_t e) {
+        float *c = S.cont + e;
+        const F3 rd_in = mk3(c[0], c[stride], c[2 * stride]);
+        const F3 lrd_in = mk3(c[3 * stride], c[4 * stride], c[5 * stride]);
+        F3 srd, w1, w2;
+        scatter_phase(K, P, mie, rd_in, lrd_in, c[6 * stride], c[7 * stride], srd, w1, w2);
+        c[0] = srd.x; c[stride] = srd.y; c[2 * stride] = srd.z;
+        c[3 * stride] = w1.x; c[4 * stride] = w1.y; c[5 * stride] = w1.z;
+        c[6 * stride] = w2.x; c[7 * stride] = w2.y; c[8 * stride] = w2.z;
+    };
+    // The escape service for lane e: its mailbox holds the escape direction,
+    // the unit's sums' index and the path throughput; the unit's sums in
+    // memory hold the path's radiance without the escape. Adds the sun's
+    // and the sky's radiance along the direction as the lane would have
+    // (path radiance += throughput * radiance, then the unit's sums, zero,
+    // + path radiance) and writes the sums of squares where they are kept.
+    auto escape_service = [&](uint32_t e) {
+        constexpr uint32_t kNb = kSumSq ? 4u : 2u;
+        const float4 *mb = W.esc_mail + 2 * ((size_t)blockIdx.x * kBlock + e);
+        const float4 a = mb[0];
+        const F3 d = mk3(a.x, a.y, a.z);
+        const F3 lc = lights_eval(K, d);
+        const F3 le = env_radiance(P.env, K, d);
+        // (The rest of the job is read after the radiance, which keeps it
+        // out of the registers the sky model needs.)
+        asm volatile("" ::: "memory");
+        const float4 b = mb[1];
+        const uint32_t dst = __float_as_uint(a.w);
+        float4 *oL, *oE, *qL, *qE;
+        if (W.partial) {
+            float4 *o = reinterpret_cast<float4 *>(W.partial) + (size_t)dst * kNb;
+            oL = o; oE = o + 1; qL = o + 2; qE = o + 3;
+        } else {
+            oL = reinterpret_cast<float4 *>(W.out_l) + dst;
+            oE = reinterpret_cast<float4 *>(W.out_e) + dst;
+            qL = reinterpret_cast<float4 *>(W.sq_l) + dst;
+            qE = reinterpret_cast<float4 *>(W.sq_e) + dst;
+        }
+        const float4 l = *oL, v = *oE;
+        const F3 pL = mk3(l.x + b.x * lc.x, l.y + b.y * lc.y, l.z + b.z * lc.z);
+        const F3 pE = mk3(v.x + b.x * le.x, v.y + b.y * le.y, v.z + b.z * le.z);
+        *oL = make_float4(0.0f + pL.x, 0.0f + pL.y, 0.0f + pL.z, l.w);
+        *oE = make_float4(0.0f + pE.x, 0.0f + pE.y, 0.0f + pE.z, v.w);
+        if (kSumSq) {
+            *qL = make_float4(0.0f + pL.x * pL.x, 0.0f + pL.y * pL.y, 0.0f + pL.z * pL.z, l.w);
+            *qE = make_float4(0.0f + pE.x * pE.x, 0.0f + pE.y * pE.y, 0.0f + pE.z * pE.z, v.w);
+        }
+    };
+    // Unclaimed jobs of queue q (wave-uniform).
+    auto queued = [&](uint32_t q, uint32_t &head) -> uint32_t {
+        head = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&S.q_head[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        const uint32_t tail = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&S.q_tail[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        return tail - head;
+    };
+    // A wave serves up to 64 queued jobs of one queue at once, claimed with
+    // one LDS compare-and-swap on the queue head: scattering jobs once
+    // kScatterBatch are queued, else escape jobs once kEscapeBatch are, or
+    // either when none of its own lanes can progress. Called with every lane
+    // active.
+    auto serve = [&](bool stalled) {
+        uint32_t h0, h1;
+        const uint32_t a0 = queued(kQueueScatter, h0), a1 = queued(kQueueEscape, h1);
+        uint32_t q, head, avail;
+        if (kScatSvc && (a0 >= kScatterBatch || (stalled && a0 > 0))) {
+            q = kQueueScatter; head = h0; avail = a0;
+        } else if (kEscSvc && (a1 >= kEscapeBatch || (stalled && a1 > 0))) {
+            q = kQueueEscape; head = h1; avail = a1;
+        } else {
+            return;
+        }
+        const uint32_t n = min(64u, avail);
+        uint32_t ok = 0;
+        if (lane_id() == 0) {
+            uint32_t expect = head;
+            ok = __hip_atomic_compare_exchange_strong(&S.q_head[q], &expect, head + n, __ATOMIC_RELAXED,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                     ? 1u : 0u;
+        }
+        if (!__builtin_amdgcn_readfirstlane(ok)) return;
+        if (lane_id() < n) {
+            const uint32_t idx = (head + lane_id()) & (kBlock - 1);
+            uint32_t e = kSlotFree;
+            for (uint32_t it = 0; it < kSpinLimit; ++it) {
+                e = __hip_atomic_load(&S.q_ring[q][idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (e != kSlotFree) break;
+            }
+            if (e != kSlotFree) {
+                __hip_atomic_store(&S.q_ring[q][idx], kSlotFree, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                if (kScatSvc && q == kQueueScatter) {
+                    scatter_service(e);
+                    __hip_atomic_fetch_or(&S.q_flag[e], kFlagScatterDone, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else if (kEscSvc) {
+                    escape_service(e);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __hip_atomic_fetch_and(&S.q_flag[e], ~kFlagEscapeBusy, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            } else {
+                W.counter[kStatusWord] = 2u;
+            }
+        }
+    };
+
     // -W * sigma_e * integral over [a, b] of the trilinear weights, by one
     // uniform point of the adjoint stream (drawn for every covered leaf with a
     // non-zero majorant, so the stream stays aligned with the oracle).
@@ -390,6 +636,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         if (W.round_limit != 0 && ++rounds > W.round_limit) {
             if (lane_id() == 0) W.counter[kStatusWord] = 1u;
             break;
+        }
+        // ---- lanes at a shadow ray's end whose scattering job is back ------
+        if (kScatSvc && st == ST_WAIT &&
+            (__hip_atomic_load(&S.q_flag[threadIdx.x], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) &
+             kFlagScatterDone)) {
+            st = ST_END;
+            svc = true;
         }
         // ---- refill lanes that finished their unit ------------------------
         // The pixels are split into W.nbands contiguous row bands; a wave
@@ -442,7 +695,12 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 r_extra = 0;
             }
         }
-        if (!__ballot(st != ST_DONE)) break;
+        // A wave leaves when its lanes are drained and no escape job waits
+        // (those have no waiting lane; the last wave to leave serves them).
+        if (!__ballot(st != ST_DONE)) {
+            uint32_t h;
+            if (!kEscSvc || queued(kQueueEscape, h) == 0) break;
+        }
 
         // ---- new camera paths and ray starts, one converged pass each -----
         if (st == ST_PATH) start_path();
@@ -594,6 +852,17 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         // carries a zero weight (what remains of the path's total), so the
         // rest of the path has no gradient to give.
         auto shadow_end = [&]() -> bool {
+            if (kScatSvc) {
+                // The service left the MIS weights: the shadow weight and the
+                // continuation's throughput from the path throughput at the
+                // event (beta is unchanged while the shadow ray is traced).
+                cont[6 * stride] = beta.x * cont[6 * stride] * albedo;
+                cont[7 * stride] = beta.y * cont[7 * stride] * albedo;
+                cont[8 * stride] = beta.z * cont[8 * stride] * albedo;
+                cont[3 * stride] = beta.x * cont[3 * stride] * albedo;
+                cont[4 * stride] = beta.y * cont[4 * stride] * albedo;
+                cont[5 * stride] = beta.z * cont[5 * stride] * albedo;
+            }
             if (!hit) {
                 F3 le = lights_eval(K, rd);
                 if (kWt) le = mk3(le.x * rw, le.y * rw, le.z * rw);
@@ -621,10 +890,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         // other's ray ends.)
         const uint32_t n_live = (uint32_t)__popcll(__ballot(st != ST_DONE));
         const uint32_t end_batch = min(kEndBatch, max(1u, n_live / 2));
+        // Lanes that waited for their scattering job end their ray at once.
         const bool run_end = (uint32_t)__popcll(__ballot(st == ST_END)) >= end_batch ||
                              !__ballot(st == ST_TRAV || st == ST_LEAF);
-        if (run_end && st == ST_END) {
+        if ((run_end || svc) && st == ST_END) {
             bool path_done = false;
+            bool esc_defer = false;   // the escape radiance goes to the escape service
+            svc = false;
             // Weighted tracking: a camera or continuation ray's weight joins
             // the path throughput before the ray's end uses it.
             if (kWt && kind != RK_SHADOW && kind != RK_SHADOW_PASS) beta = mk3(beta.x * rw, beta.y * rw, beta.z * rw);
@@ -639,14 +911,41 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 alpha = 1.0f;
                 path_done = true;
             } else if (kind == RK_SHADOW) {
-                path_done = shadow_end();
+                // The continuation and the shadow weight come from the
+                // scattering service; wait for them if they are not back.
+                if (kScatSvc && !(__hip_atomic_load(&S.q_flag[threadIdx.x], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) &
+                                kFlagScatterDone))
+                    st = ST_WAIT;
+                else
+                    path_done = shadow_end();
             } else if (!hit) {
                 if (depth_r > 0 && ((kAdj && pass == 0) || kRec)) last_c = (int32_t)depth_r;
                 if (depth_r > 0) {
-                    F3 lc = lights_eval(K, rd);
-                    F3 le = env_radiance(P.env, K, rd);
-                    pathL.x += beta.x * lc.x; pathL.y += beta.y * lc.y; pathL.z += beta.z * lc.z;
-                    pathE.x += beta.x * le.x; pathE.y += beta.y * le.y; pathE.z += beta.z * le.z;
+                    // The escape service takes the radiance when this path
+                    // ends a unit whose sums are zero so far and the lane's
+                    // previous escape job has been taken.
+                    if (kEscSvc && W.esc_mail && s_cur + 1 == s_end && accA == 0.0f && accL[0] == 0.0f &&
+                        accL[1] == 0.0f && accL[2] == 0.0f && accE[0] == 0.0f && accE[1] == 0.0f &&
+                        accE[2] == 0.0f &&
+                        !(__hip_atomic_load(&S.q_flag[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
+                          kFlagEscapeBusy)) {
+                        esc_defer = true;
+                        uint32_t dst;
+                        if (W.partial) {
+                            dst = chunk_cur * W.npix + pix_local;
+                        } else {
+                            const uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
+                            dst = ly * P.tilex + lx;
+                        }
+                        float4 *mb = W.esc_mail + 2 * ((size_t)blockIdx.x * kBlock + threadIdx.x);
+                        mb[0] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(dst));
+                        mb[1] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                    } else {
+                        F3 lc = lights_eval(K, rd);
+                        F3 le = env_radiance(P.env, K, rd);
+                        pathL.x += beta.x * lc.x; pathL.y += beta.y * lc.y; pathL.z += beta.z * lc.z;
+                        pathE.x += beta.x * le.x; pathE.y += beta.y * le.y; pathE.z += beta.z * le.z;
+                    }
                 }
                 path_done = true;
             } else {
@@ -662,33 +961,35 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 if (depth_r >= P.scattevs) {
                     path_done = true;
                 } else {
-                    // Scattering event: MIS between the phase function and lights[0]
-                    // (src/kernel.cpp:387-430).
+                    // Scattering event (src/kernel.cpp:387-430): the
+                    // continuation waits in cont[] for the shadow ray's end.
                     event = depth_r;
-                    U4 r4 = philox4x32_10(U4{P.sample_base + s_cur, event, 0u, kRngScatter}, key0, P.key1);
-                    float u1 = u01(r4.x), u2 = u01(r4.y), u3 = u01(r4.z), u4 = u01(r4.w);
-                    F3 srd, p1, p3;
-                    F3 lrd = sun_sample(K, u3, u4);
-                    if (P.phase == DRK_PHASE_MIE) {
-                        srd = mie_sample(mie, rd, u1, u2);
-                        p1 = mie_eval_rgb(mie, dot3(srd, rd));
-                        p3 = mie_eval_rgb(mie, dot3(lrd, rd));
+                    F3 lrd, shadow_w;
+                    if (kScatSvc) {
+                        shadow_w = mk3(0, 0, 0);   // (used by the adjoint only, which does not queue)
+                        // The shadow ray's direction now; the rest of the
+                        // event from the workgroup service, meanwhile.
+                        float u1, u2;
+                        scatter_uniforms(K, P, key0, P.sample_base + s_cur, event, u1, u2, lrd);
+                        cont[0] = rd.x; cont[stride] = rd.y; cont[2 * stride] = rd.z;
+                        cont[3 * stride] = lrd.x; cont[4 * stride] = lrd.y; cont[5 * stride] = lrd.z;
+                        cont[6 * stride] = u1; cont[7 * stride] = u2;
+                        __hip_atomic_fetch_and(&S.q_flag[threadIdx.x], ~kFlagScatterDone, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        queue_push(kQueueScatter);
                     } else {
-                        srd = hg_sample(P.hg_g, rd, u1, u2);
-                        float h1 = hg_eval(P.hg_g, dot3(srd, rd)), h3 = hg_eval(P.hg_g, dot3(lrd, rd));
-                        p1 = mk3(h1, h1, h1);
-                        p3 = mk3(h3, h3, h3);
+                        F3 srd, w1, w2;
+                        float u1, u2;
+                        scatter_uniforms(K, P, key0, P.sample_base + s_cur, event, u1, u2, lrd);
+                        scatter_phase(K, P, mie, rd, lrd, u1, u2, srd, w1, w2);
+                        shadow_w = mk3(beta.x * w2.x * albedo, beta.y * w2.y * albedo, beta.z * w2.z * albedo);
+                        cont[0] = srd.x; cont[stride] = srd.y; cont[2 * stride] = srd.z;
+                        cont[3 * stride] = beta.x * w1.x * albedo;
+                        cont[4 * stride] = beta.y * w1.y * albedo;
+                        cont[5 * stride] = beta.z * w1.z * albedo;
+                        cont[6 * stride] = shadow_w.x; cont[7 * stride] = shadow_w.y; cont[8 * stride] = shadow_w.z;
                     }
-                    float pd1 = sun_pdf(K, srd), pd2 = sun_pdf(K, lrd);
-                    F3 w1 = mk3(div_approx(p1.x, p1.x + pd1), div_approx(p1.y, p1.y + pd1), div_approx(p1.z, p1.z + pd1));
-                    F3 w2 = mk3(div_approx(p3.x, p3.x + pd2), div_approx(p3.y, p3.y + pd2), div_approx(p3.z, p3.z + pd2));
                     F3 x = mk3(ro.x + rd.x * td, ro.y + rd.y * td, ro.z + rd.z * td);
-                    const F3 shadow_w = mk3(beta.x * w2.x * albedo, beta.y * w2.y * albedo, beta.z * w2.z * albedo);
-                    cont[0] = srd.x; cont[stride] = srd.y; cont[2 * stride] = srd.z;
-                    cont[3 * stride] = beta.x * w1.x * albedo;
-                    cont[4 * stride] = beta.y * w1.y * albedo;
-                    cont[5 * stride] = beta.z * w1.z * albedo;
-                    cont[6 * stride] = shadow_w.x; cont[7 * stride] = shadow_w.y; cont[8 * stride] = shadow_w.z;
                     ro = x;
                     rd = lrd;
                     kind = RK_SHADOW;
@@ -781,9 +1082,29 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                             reinterpret_cast<float4 *>(W.sq_e)[d] = make_float4(sqE[0], sqE[1], sqE[2], accA);
                         }
                     }
+                    if (kEscSvc && esc_defer) {
+                        // The sums above lack the escape radiance: queue the
+                        // job that adds it (after the sums and the mailbox
+                        // are written).
+                        __hip_atomic_fetch_or(&S.q_flag[threadIdx.x], kFlagEscapeBusy, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                        queue_push(kQueueEscape);
+                    }
                     st = ST_WORK;
                 }
             }
+        }
+
+        // ---- workgroup services ------------------------------------------
+        if (kScatSvc || kEscSvc) {
+            const bool stalled = !__ballot(st != ST_WAIT && st != ST_DONE);
+            stall_rounds = stalled ? stall_rounds + 1u : 0u;
+            if (stall_rounds > kStallLimit) {
+                // A job did not come back: leave with the status word set.
+                if (lane_id() == 0) W.counter[kStatusWord] = 2u;
+                if (st == ST_WAIT) st = ST_DONE;
+            }
+            serve(stalled);
         }
     }
     if (kAdj && W.ev_cap) {
@@ -809,7 +1130,7 @@ template <int kMode>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render(RenderParams P, WorkParams W) {
     __shared__ RenderLds S;
     render_lds_init(S);
-    render_body<kMode>(P, W, S);
+    render_body<kMode, true>(P, W, S);
 }
 
 // The adjoint, like the forward, at four waves per SIMD: its image gradient
@@ -819,7 +1140,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                                                                     WorkParams W) {
     __shared__ RenderLds S;
     render_lds_init(S);
-    render_body<kModeAdjoint>(P, W, S);
+    render_body<kModeAdjoint, false>(P, W, S);
 }
 
 // The adjoint with null-collision terms (the pathwise derivative of the
@@ -828,7 +1149,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                                                                          WorkParams W) {
     __shared__ RenderLds S;
     render_lds_init(S);
-    render_body<kModeAdjointNull>(P, W, S);
+    render_body<kModeAdjointNull, false>(P, W, S);
 }
 
 // Many launches (scenes, views, tiles) in one persistent grid. Workgroup b
@@ -844,7 +1165,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const uint32_t j0 = blockIdx.x % njobs;
     for (uint32_t i = 0; i < njobs; ++i) {
         const uint32_t j = (j0 + i < njobs) ? j0 + i : j0 + i - njobs;
-        render_body<kMode>(Ps[j], Ws[j], S);
+        render_body<kMode, false>(Ps[j], Ws[j], S);
     }
 }
 

@@ -64,6 +64,8 @@ struct WorkParams {
     uint32_t nchunks;
     float *partial;           // [nchunks][npix][buffers] float4 when nchunks > 1, else nullptr: a unit's
                               // buffers side by side, written by one lane as one 32/64-byte span
+    float4 *esc_mail;         // per lane of the grid two float4: the escape service's mailbox (nullptr:
+                              // escape radiance computed in place)
     float *out_l, *out_e;     // tile float4 outputs (stride tilex)
     float *sq_l, *sq_e;       // sum-of-squares outputs or nullptr
     unsigned long long *counts;  // [node touches, brick lookups] for the counting pass

@@ -783,9 +783,10 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         }
         W.partial = k->d_partial;
     }
-    // Mailboxes of the escape service (forward modes without the path
-    // record; the allocation-free entry computes escape radiance in place).
-    if (!noalloc && (mode == kModeForward || mode == kModeForwardSq || mode == kModeCount || mode == kModeWeighted)) {
+    // Mailboxes of the escape service (forward modes; the allocation-free
+    // entry computes escape radiance in place).
+    if (!noalloc && (mode == kModeForward || mode == kModeForwardSq || mode == kModeCount || mode == kModeWeighted ||
+                     mode == kModeRecord)) {
         const size_t need = (size_t)grid * render_block_size() * 32;
         if (need > k->mail_bytes) {
             if (k->d_mail) (void)hipFree(k->d_mail);

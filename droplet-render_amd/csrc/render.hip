@@ -263,9 +263,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     // The adjoint needs the shadow ray's weight when the ray starts, so it
     // samples its scattering events in place.
     static_assert(!(kQueue && kAdj), "the adjoint does not use the scattering service");
-    // The escape service writes the unit's sums itself, which the path
-    // record (written at the path's end) does not allow.
-    constexpr bool kEscSvc = kQueue && !kRec && DRK_ESCAPE_SERVICE;
+    constexpr bool kEscSvc = kQueue && DRK_ESCAPE_SERVICE;
     constexpr bool kScatSvc = kQueue && DRK_SCATTER_SERVICE;
     MieLds &mie = S.mie;
     unsigned long long (*ev_cur)[2] = S.ev_cur;
@@ -521,8 +519,9 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         c[6 * stride] = w2.x; c[7 * stride] = w2.y; c[8 * stride] = w2.z;
     };
     // The escape service for lane e: its mailbox holds the escape direction,
-    // the unit's sums' index and the path throughput; the unit's sums in
-    // memory hold the path's radiance without the escape. Adds the sun's
+    // the unit's sums' index, the path throughput and (record mode) the
+    // record index; the unit's sums (and the record) in memory hold the
+    // path's radiance without the escape. Adds the sun's
     // and the sky's radiance along the direction as the lane would have
     // (path radiance += throughput * radiance, then the unit's sums, zero,
     // + path radiance) and writes the sums of squares where they are kept.
@@ -551,6 +550,15 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         const float4 l = *oL, v = *oE;
         const F3 pL = mk3(l.x + b.x * lc.x, l.y + b.y * lc.y, l.z + b.z * lc.z);
         const F3 pE = mk3(v.x + b.x * le.x, v.y + b.y * le.y, v.z + b.z * le.z);
+        if (kRec) {
+            // The path record holds the path's radiance as well (index in
+            // the mailbox's fourth word).
+            float4 *R = reinterpret_cast<float4 *>(W.rec);
+            const size_t n = (size_t)W.npix * P.samples, ri = __float_as_uint(b.w);
+            const float4 r0 = R[ri], r1 = R[n + ri];
+            R[ri] = make_float4(r0.x + b.x * lc.x, r0.y + b.y * lc.y, r0.z + b.z * lc.z, r0.w);
+            R[n + ri] = make_float4(r1.x + b.x * le.x, r1.y + b.y * le.y, r1.z + b.z * le.z, r1.w);
+        }
         *oL = make_float4(0.0f + pL.x, 0.0f + pL.y, 0.0f + pL.z, l.w);
         *oE = make_float4(0.0f + pE.x, 0.0f + pE.y, 0.0f + pE.z, v.w);
         if (kSumSq) {
@@ -925,6 +933,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     // ends a unit whose sums are zero so far and the lane's
                     // previous escape job has been taken.
                     if (kEscSvc && W.esc_mail && s_cur + 1 == s_end && accA == 0.0f && accL[0] == 0.0f &&
+                        (!kRec || (uint64_t)s_cur * W.npix + pix_local <= 0xffffffffull) &&
                         accL[1] == 0.0f && accL[2] == 0.0f && accE[0] == 0.0f && accE[1] == 0.0f &&
                         accE[2] == 0.0f &&
                         !(__hip_atomic_load(&S.q_flag[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
@@ -939,7 +948,8 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         }
                         float4 *mb = W.esc_mail + 2 * ((size_t)blockIdx.x * kBlock + threadIdx.x);
                         mb[0] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(dst));
-                        mb[1] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                        mb[1] = make_float4(beta.x, beta.y, beta.z,
+                                            __uint_as_float(kRec ? s_cur * W.npix + pix_local : 0u));
                     } else {
                         F3 lc = lights_eval(K, rd);
                         F3 le = env_radiance(P.env, K, rd);

@@ -106,3 +106,40 @@ def test_batch_rejects_bad_jobs(pkg, gpu):
     with pytest.raises(L.DrkError):
         pkg.batch.Batch([(k, 0, 0, 16, 16, 0, 1, 0, big, big)])     # no samples
     k.Destroy()
+
+
+@pytest.mark.parametrize("chunk", [1, 3])
+def test_ray_end_services_change_no_bits(pkg, gpu, chunk):
+    """The single-launch kernel runs the scattering phase and the escape
+    radiance as workgroup services (an escaping lane's radiance is added to
+    its unit's sums in memory by another wave); the batched kernel and the
+    allocation-free entry point run them in place. Sums and sums of squares
+    agree bit for bit, with single-sample units (every escape after
+    scattering deferred) and with chunks of 3 (only escapes that end a unit
+    whose sums are still zero are deferred); so do the sums of the forward
+    that keeps the path record."""
+    torch = gpu
+    wl = pkg.configs.tiny(n=40, w=48, h=40, spp=12, scattevs=6, seed=7)
+    k = wl.kernel()
+    k.set_tuning(chunk=chunk)
+    n = wl.w * wl.h * 4
+
+    def bufs():
+        return [torch.full((n,), -1.0, device="cuda") for _ in range(4)]
+    single, noalloc, batched = bufs(), bufs(), bufs()
+    k.render_device(0, 0, wl.w, wl.h, wl.spp, 123, 0, *single)
+    k.render_device_noalloc(0, 0, wl.w, wl.h, wl.spp, 123, 0, *noalloc)
+    b = pkg.batch.Batch([(k, 0, 0, wl.w, wl.h, wl.spp, 123, 0, *batched)])
+    b.launch()
+    rec = torch.empty(k.record_bytes(0, 0, wl.w, wl.h, wl.spp) // 4, device="cuda")
+    rl, re = torch.empty(n, device="cuda"), torch.empty(n, device="cuda")
+    k.render_record_device(0, 0, wl.w, wl.h, wl.spp, 123, 0, rl, re, rec)
+    torch.cuda.synchronize()
+    for i, (a, c, d) in enumerate(zip(single, batched, noalloc)):
+        assert torch.equal(a, c), ("batched", i)
+        assert torch.equal(a, d), ("noalloc", i)
+    assert torch.equal(rl, single[0]) and torch.equal(re, single[1])
+    a = single[0].view(-1, 4)
+    assert float(a[:, 3].sum()) > 0 and float(single[1].view(-1, 4)[:, :3].sum()) > 0
+    b.destroy()
+    k.Destroy()

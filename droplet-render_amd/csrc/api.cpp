@@ -669,6 +669,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     W.grad_nxcc = k->xcc;
     W.nbands = k->bands;
     W.npix = npix;
+    set_bands(W, rx);
     W.chunk = chunk;
     W.nfull = nfull;
     W.nchunks = nchunks;
@@ -1182,6 +1183,7 @@ static int batch_plan(drk_batch *b) {
         W.round_limit = k->round_limit;
         W.nbands = k->bands;
         W.npix = npix;
+        set_bands(W, rx);
         W.chunk = chunk;
         W.nfull = nfull;
         W.nchunks = nchunks;

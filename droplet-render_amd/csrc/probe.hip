@@ -10,6 +10,7 @@
 #include "drk_internal.h"
 #include "droplet_render_probe.h"
 #include "estimator.h"
+#include "render_launch.h"
 
 namespace drk {
 namespace {
@@ -466,6 +467,17 @@ int drk_probe_map_env(const float *tex, uint32_t w, uint32_t h, const float *rd,
     k_map_env<<<blocks(n), kProbeBlock>>>(dtex, w, h, drd, n, dout);
     PTRY(hipGetLastError(), "probe launch");
     PTRY(hipMemcpy(out_rgb, dout, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
+    return DRK_OK;
+}
+
+int drk_probe_udiv(uint32_t d, uint32_t n0, uint32_t step, uint64_t count, uint64_t *mismatches) {
+    if (!mismatches || d == 0) return set_error(DRK_EINVAL, "drk_probe_udiv: NULL argument or zero divisor");
+    const UDiv D = udiv_make(d);
+    uint64_t bad = 0;
+    uint32_t n = n0;
+    for (uint64_t i = 0; i < count; ++i, n += step) bad += udiv(n, D) != n / d;
+    for (uint32_t j = 0; j < 64; ++j) bad += udiv(0xffffffffu - j, D) != (0xffffffffu - j) / d;
+    *mismatches = bad;
     return DRK_OK;
 }
 

@@ -280,7 +280,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     // Work unit.
     uint32_t st = ST_WORK;
     uint32_t chunk_cur = 0, pix_local = 0, key0 = 0;   // key0: frame pixel index
-    const uint32_t band0 = xcc_id();   // first band: this wave's XCD
+    const uint32_t band0 = xcc_id() % W.nbands;   // first band: this wave's XCD's
     uint32_t band_step = 0;            // bands this wave has found drained (wave-uniform)
     // The wave's reserve of fetched units of the current band, [r_base,
     // r_base + r_cnt), and how many extra units its next fetch asks for
@@ -661,16 +661,16 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 if (need) st = ST_DONE;   // every band drained
                 break;
             }
-            const uint32_t b = (band0 + band_step) % W.nbands;
-            const uint32_t p0 = (uint32_t)((uint64_t)b * W.npix / W.nbands);
-            const uint32_t bp = (uint32_t)((uint64_t)(b + 1) * W.npix / W.nbands) - p0;
+            const uint32_t b = band0 + band_step < W.nbands ? band0 + band_step : band0 + band_step - W.nbands;
+            const uint32_t p0 = W.band_p0[b];
+            const uint32_t bp = W.band_p0[b + 1] - p0;
             const uint32_t total = bp * W.nchunks;
             const uint32_t u = take_units(W.counter + kCounterStride * b, need, bp, W.nfull, total, r_base, r_cnt,
                                           r_extra);
             if (need && u < total) {
-                chunk_cur = u / bp;
+                chunk_cur = udiv(u, W.band_div[b]);
                 pix_local = p0 + (u - chunk_cur * bp);
-                uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
+                const uint32_t ly = udiv(pix_local, W.rx_div), lx = pix_local - ly * P.rx;
                 key0 = (P.y0 + ly) * P.w + P.x0 + lx;
                 s_cur = chunk_start(W.chunk, W.nfull, P.samples, chunk_cur);
                 s_end = chunk_start(W.chunk, W.nfull, P.samples, chunk_cur + 1);
@@ -943,7 +943,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         if (W.partial) {
                             dst = chunk_cur * W.npix + pix_local;
                         } else {
-                            const uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
+                            const uint32_t ly = udiv(pix_local, W.rx_div), lx = pix_local - ly * P.rx;
                             dst = ly * P.tilex + lx;
                         }
                         float4 *mb = W.esc_mail + 2 * ((size_t)blockIdx.x * kBlock + threadIdx.x);
@@ -1083,7 +1083,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                             oP[3] = make_float4(sqE[0], sqE[1], sqE[2], accA);
                         }
                     } else {
-                        uint32_t lx = pix_local % P.rx, ly = pix_local / P.rx;
+                        const uint32_t ly = udiv(pix_local, W.rx_div), lx = pix_local - ly * P.rx;
                         size_t d = (size_t)ly * P.tilex + lx;
                         reinterpret_cast<float4 *>(W.out_l)[d] = make_float4(accL[0], accL[1], accL[2], accA);
                         reinterpret_cast<float4 *>(W.out_e)[d] = make_float4(accE[0], accE[1], accE[2], accA);

@@ -13,6 +13,33 @@ namespace drk {
 // chunk u / bp, pixel b*npix/nbands + u % bp.
 constexpr uint32_t kMaxBands = 16;
 
+// Unsigned division by an invariant 32-bit divisor d >= 1 with one
+// multiply-high (Granlund and Montgomery, "Division by invariant integers
+// using multiplication", PLDI 1994, Fig. 4.1): with l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n), the quotient
+// floor(n / d) is (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0) for every
+// 32-bit n.
+struct UDiv {
+    uint32_t m, s1, s2;
+};
+inline UDiv udiv_make(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) ++l;
+    UDiv r;
+    r.m = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    r.s1 = l < 1 ? l : 1u;
+    r.s2 = l > 0 ? l - 1 : 0u;
+    return r;
+}
+__host__ __device__ inline uint32_t udiv(uint32_t n, const UDiv &D) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t t = __umulhi(D.m, n);
+#else
+    const uint32_t t = (uint32_t)(((uint64_t)D.m * n) >> 32);
+#endif
+    return (t + ((n - t) >> D.s1)) >> D.s2;
+}
+
 // First sample of chunk k, k <= nchunks. Chunks below nfull hold `chunk`
 // samples (the last of them fewer when the samples run out first); each
 // chunk after them takes half, rounded up, of the samples that remain, down
@@ -57,6 +84,9 @@ struct WorkParams {
                               // and the status word at kStatusWord (cleared per launch)
     uint32_t round_limit;     // rounds of the state machine a wave may run in this launch (0: no limit)
     uint32_t nbands;          // row bands of the tile's pixels (1..kMaxBands)
+    uint32_t band_p0[kMaxBands + 1];   // first pixel of each band (band_p0[nbands] = npix)
+    UDiv band_div[kMaxBands];          // division by each band's pixel count
+    UDiv rx_div;                       // division by the rendered extent's width (rx)
     uint32_t total_units;     // npix * nchunks
     uint32_t npix;            // rx * ry
     uint32_t chunk;           // samples per unit in the leading chunks
@@ -103,6 +133,17 @@ inline uint32_t partial_buffers(int mode) { return (mode == kModeForwardSq || mo
 
 // Bytes of one (pixel, sample) entry of the path record.
 constexpr size_t kRecordBytes = 40;
+
+// Fills W's band table and divisors for W.npix pixels in rows of rx and
+// W.nbands bands.
+inline void set_bands(WorkParams &W, uint32_t rx) {
+    for (uint32_t b = 0; b <= W.nbands; ++b) W.band_p0[b] = (uint32_t)((uint64_t)b * W.npix / W.nbands);
+    for (uint32_t b = 0; b < W.nbands; ++b) {
+        const uint32_t bp = W.band_p0[b + 1] - W.band_p0[b];
+        W.band_div[b] = udiv_make(bp ? bp : 1u);   // (an empty band is never divided by)
+    }
+    W.rx_div = udiv_make(rx ? rx : 1u);
+}
 
 int render_block_size();
 int render_blocks_per_cu(int mode);

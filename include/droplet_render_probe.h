@@ -63,6 +63,12 @@ int drk_probe_traverse(const drk_scene *scene, const float *ro, const float *rd,
 int drk_probe_camera(const float viewi[16], const float proji[16], uint32_t w, uint32_t h, const uint32_t *px,
                      const uint32_t *py, uint32_t n, float *ro, float *rd, uint32_t *gm);
 
+/* Host check of the kernels' division by an invariant divisor (the work-unit
+ * to pixel mapping): counts the n in {n0 + i*step mod 2^32 : i < count} and
+ * the 64 largest 32-bit values whose quotient by d differs from n / d.
+ * Runs on the CPU (no device needed). */
+int drk_probe_udiv(uint32_t d, uint32_t n0, uint32_t step, uint64_t count, uint64_t *mismatches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,3 +61,29 @@ def test_guided_tail_halves_down_to_one(schedules):
             rem -= z
         assert tail == want, (s, c, tail, want)
         assert tail[-1] == 1
+
+
+def test_invariant_division_is_exact(pkg):
+    """The kernels map a work unit to its chunk and pixel with a
+    multiply-high division by the band's pixel count and by the row width
+    (UDiv, render_launch.h); its quotients equal n / d for every divisor
+    shape (1, powers of two and their neighbours, the largest values, the C1
+    band and row sizes, random ones) over dividends spread across the whole
+    32-bit range, exhaustively near 0 and at the top."""
+    import ctypes as C
+    import numpy as np
+    L = pkg._lib.lib()
+    rng = np.random.default_rng(11)
+    ds = [1, 2, 3, 5, 7, 10, 641, 1023, 1024, 1025, 65535, 65536, 65537, 73728, 589824, 1 << 24,
+          (1 << 31) - 1, 1 << 31, (1 << 31) + 1, 0xFFFFFFFE, 0xFFFFFFFF]
+    ds += [int(x) for x in rng.integers(1, 1 << 32, 40, dtype=np.uint64)]
+    ds += [int(x) for x in rng.integers(1, 1 << 16, 20)]
+    bad = C.c_uint64()
+    for d in ds:
+        assert L.drk_probe_udiv(d, 0, 1, 1 << 18, C.byref(bad)) == 0 and bad.value == 0, d
+        # strided over all of [0, 2^32): an odd step visits dividends of every magnitude
+        n0 = int(rng.integers(0, 1 << 32))
+        assert L.drk_probe_udiv(d, n0, 2654435761, 1 << 20, C.byref(bad)) == 0 and bad.value == 0, d
+        # multiples of d and their neighbours
+        assert L.drk_probe_udiv(d, d - 1, d, min(1 << 16, (1 << 32) // d), C.byref(bad)) == 0 and bad.value == 0, d
+    assert L.drk_probe_udiv(0, 0, 1, 1, C.byref(bad)) != 0

@@ -79,6 +79,10 @@ struct drk_scene {
     uint32_t *d_cells = nullptr;
     uint32_t grid_levels = 0;
     float grid_occupancy = 0.0f;
+    // RN(1 / e) when every leaf has the same half-extent e (the reference's
+    // build puts all leaves at one depth, each level halving e exactly), for
+    // the lookups' division by e; 0 otherwise.
+    float leaf_rcp = 0.0f;
     uint32_t walk = DRK_WALK_AUTO;
     uint32_t node_count = 0;
     uint32_t leafx[2] = {0, 0};
@@ -267,6 +271,21 @@ int drk_scene_create(const drk_octree_node *ob, uint32_t node_count, const float
     if (!s) return set_error(DRK_ENOMEM, "drk_scene_create: host allocation");
     s->grid_levels = grid_levels;
     s->grid_occupancy = grid_occ;
+    {
+        float e = 0.0f;
+        bool have = false, same = true;
+        for (uint32_t i = 0; i < node_count && same; ++i) {
+            const drk_octree_node &n = ob[i];
+            if (n.volx[0] == DRK_NO_BRICK && n.volx[1] == DRK_NO_BRICK) continue;
+            if (!have) e = n.ce[3];
+            have = true;
+            same = n.ce[3] == e;
+        }
+        // Markstein's correction needs a normal reciprocal and normal
+        // quotients: the lookups use it for numerators of at least 2^-100,
+        // whose quotients stay normal for extents up to 2^20.
+        if (have && same && e >= 0x1p-60f && e <= 0x1p20f) s->leaf_rcp = 1.0f / e;
+    }
     if (const char *w = std::getenv("DRK_WALK")) s->walk = (uint32_t)std::atoi(w);
     s->node_count = node_count;
     s->h_ob.assign(ob, ob + node_count);
@@ -456,6 +475,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     P.pvol_sdf = scene->d_sdf;
     P.lvoxc = scene->lvoxc;
     P.lvoxc3 = scene->lvoxc * scene->lvoxc * scene->lvoxc;
+    P.leaf_rcp = scene->leaf_rcp;
     P.w = w;
     P.h = h;
     P.scattevs = scattevs;

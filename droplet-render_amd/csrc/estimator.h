@@ -413,13 +413,40 @@ __device__ __forceinline__ int grid_next(const uint32_t *__restrict__ cells, uin
 // lvoxc^3 vertex-centred brick, lerping x, then y, then z; lerp(a,b,t) is
 // t*(b-a)+a with separate rounding, as the reference's SSE3 build computes it.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float brick_lookup(const float *__restrict__ brick, F3 p, float4 ce, uint32_t lvoxc) {
+// The lookup's cell coordinate a/e along one axis, a = (p - (c - e))*(n-1)/2
+// as the reference writes it and e the leaf's half-extent, correctly rounded
+// either way: by the division, or, given the correctly rounded reciprocal
+// y = RN(1/e) (rcp != 0: every leaf of the scene has the same extent), by
+// Markstein's correction of q = RN(a*y): the residual r = a - e*q is exact in
+// one fma, and RN(q + r*y) = RN(a/e) when y is within half an ulp of 1/e and q
+// within an ulp of a/e, unless the residual underflows (P. Markstein, "Computation
+// of elementary functions on the IBM RISC System/6000 processor", IBM J. Res.
+// Dev. 34(1), 1990). It underflows only for |a| below about 2^-103, so
+// numerators under 2^-100 (zero included, which keeps its sign) take the
+// division. Three fma-class operations per axis instead of a division's ten.
+__device__ __forceinline__ float markstein_div(float a, float e, float y) {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-e, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
+__device__ __forceinline__ float brick_lookup(const float *__restrict__ brick, F3 p, float4 ce, uint32_t lvoxc,
+                                              float rcp) {
 #pragma clang fp contract(off)
     const float nv = (float)lvoxc;
     const float nvm = nv - 1.0f;
-    float ni[3] = {((-0.5f * ((ce.x - ce.w) - p.x)) * nvm) / ce.w,
-                   ((-0.5f * ((ce.y - ce.w) - p.y)) * nvm) / ce.w,
-                   ((-0.5f * ((ce.z - ce.w) - p.z)) * nvm) / ce.w};
+    const float ax = (-0.5f * ((ce.x - ce.w) - p.x)) * nvm, ay = (-0.5f * ((ce.y - ce.w) - p.y)) * nvm,
+                az = (-0.5f * ((ce.z - ce.w) - p.z)) * nvm;
+    float ni[3];
+    if (rcp != 0.0f && fminf(fminf(fabsf(ax), fabsf(ay)), fabsf(az)) >= 0x1p-100f) {
+        ni[0] = markstein_div(ax, ce.w, rcp);
+        ni[1] = markstein_div(ay, ce.w, rcp);
+        ni[2] = markstein_div(az, ce.w, rcp);
+    } else {
+        ni[0] = ax / ce.w;
+        ni[1] = ay / ce.w;
+        ni[2] = az / ce.w;
+    }
     uint32_t ia[3], ib[3];
     float nl[3];
 #pragma unroll

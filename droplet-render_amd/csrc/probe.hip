@@ -125,9 +125,10 @@ __global__ void k_sky(SkyState S, const float *rd, uint32_t n, float *out) {
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }
 
-__global__ void k_trilinear(const float *brick, uint32_t lvoxc, float4 ce, const float *p, uint32_t n, float *out) {
+__global__ void k_trilinear(const float *brick, uint32_t lvoxc, float4 ce, float rcp, const float *p, uint32_t n,
+                            float *out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = brick_lookup(brick, mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), ce, lvoxc);
+    if (i < n) out[i] = brick_lookup(brick, mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), ce, lvoxc, rcp);
 }
 
 __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node *ob, const uint32_t *chx,
@@ -381,7 +382,8 @@ int drk_probe_sky(const drk_sun *sun, float turbidity, float albedo, const float
     return DRK_OK;
 }
 
-int drk_probe_trilinear(const float *brick, uint32_t lvoxc, const float ce[4], const float *p, uint32_t n, float *out) {
+int drk_probe_trilinear(const float *brick, uint32_t lvoxc, const float ce[4], const float *p, uint32_t n,
+                        int leaf_rcp, float *out) {
     if (!brick || !ce || !p || !out) return set_error(DRK_EINVAL, "drk_probe_trilinear: NULL argument");
     if (lvoxc < 2 || lvoxc > 256) return set_error(DRK_EINVAL, "drk_probe_trilinear: lvoxc in [2,256]");
     if (!(ce[3] > 0.0f)) return set_error(DRK_EINVAL, "drk_probe_trilinear: extent must be positive");
@@ -391,7 +393,10 @@ int drk_probe_trilinear(const float *brick, uint32_t lvoxc, const float ce[4], c
     PTRY(B.upload(db, brick, (size_t)lvoxc * lvoxc * lvoxc), "probe upload");
     PTRY(B.upload(dp, p, 3 * (size_t)n), "probe upload");
     PTRY(B.alloc(dout, n), "probe alloc");
-    k_trilinear<<<blocks(n), kProbeBlock>>>(db, lvoxc, make_float4(ce[0], ce[1], ce[2], ce[3]), dp, n, dout);
+    // leaf_rcp: the division by the extent through its reciprocal, as the
+    // render kernels do for scenes whose leaves share one extent.
+    const float rcp = leaf_rcp && ce[3] >= 0x1p-60f && ce[3] <= 0x1p20f ? 1.0f / ce[3] : 0.0f;
+    k_trilinear<<<blocks(n), kProbeBlock>>>(db, lvoxc, make_float4(ce[0], ce[1], ce[2], ce[3]), rcp, dp, n, dout);
     PTRY(hipGetLastError(), "probe launch");
     PTRY(hipMemcpy(out, dout, 4 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
     return DRK_OK;

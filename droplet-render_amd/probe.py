@@ -88,10 +88,13 @@ def sky(direction, turbidity, albedo, rd):
     return out
 
 
-def trilinear(brick, lvoxc, ce, p):
+def trilinear(brick, lvoxc, ce, p, leaf_rcp=True):
+    """leaf_rcp: divide by the extent through its reciprocal (the kernels'
+    form when all leaves share one extent), else by the division."""
     brick, ce, p = _f(brick), _f(ce), _f(p).reshape(-1, 3)
     out = np.zeros(len(p), np.float32)
-    L.check(L.lib().drk_probe_trilinear(L.fptr(brick), int(lvoxc), L.fptr(ce), L.fptr(p), len(p), L.fptr(out)))
+    L.check(L.lib().drk_probe_trilinear(L.fptr(brick), int(lvoxc), L.fptr(ce), L.fptr(p), len(p),
+                                        int(bool(leaf_rcp)), L.fptr(out)))
     return out
 
 

@@ -49,9 +49,12 @@ int drk_probe_map_env(const float *tex, uint32_t w, uint32_t h, const float *rd,
 /* Hosek state in double, as the host computes it (configs 3x9, radiances 3). */
 int drk_probe_hosek_state(double turbidity, double albedo, double elevation, double *cfg27, double *rad3);
 
-/* SampleVoxelSpace: trilinear lookup of points p (3n) in one brick with node ce. */
+/* SampleVoxelSpace: trilinear lookup of points p (3n) in one brick with node
+ * ce; leaf_rcp != 0 divides by the extent through its correctly rounded
+ * reciprocal and one Markstein correction (the render kernels' form for
+ * scenes whose leaves share one extent), 0 by the division. */
 int drk_probe_trilinear(const float *brick, uint32_t lvoxc, const float ce[4], const float *p, uint32_t n,
-                        float *out);
+                        int leaf_rcp, float *out);
 
 /* Leaf sequence of each ray through the scene's octree: up to max_leaves
  * (node, tr0, tr1) per ray; count[i] is the full sequence length (may exceed

@@ -124,13 +124,31 @@ def test_sky_within_1e5_of_fp64(pkg, gpu):
     assert np.array_equal(np.isfinite(g), np.isfinite(ref))
 
 
-def test_trilinear_bit_exact(pkg, oracle, gpu):
+@pytest.mark.parametrize("leaf_rcp", [False, True])
+def test_trilinear_bit_exact(pkg, oracle, gpu, leaf_rcp):
+    """The lookup with the division by the leaf extent, and with the
+    reciprocal and Markstein correction the kernels use when all leaves share
+    one extent: bit for bit the oracle's, also for points on and next to the
+    cell planes (numerators 0 and near it, where the division is kept) and
+    extents that are not powers of two."""
     rng = np.random.default_rng(7)
     for lvoxc in (8, 16):
         brick = rng.random(lvoxc ** 3).astype(np.float32)
-        ce = np.array([0.41, 0.23, -0.6, 0.0625], np.float32)
-        p = (ce[:3] - ce[3] * 1.01 + rng.random((5000, 3)) * 2.02 * ce[3]).astype(np.float32)
-        assert np.array_equal(pkg.probe.trilinear(brick, lvoxc, ce, p), oracle.voxel(brick, p, ce, lvoxc))
+        for ce in (np.array([0.41, 0.23, -0.6, 0.0625], np.float32),
+                   np.array([0.0190, 0.0190, 0.5, 0.0190], np.float32),
+                   np.array([1.3, -2.7, 0.77, 0.3109712], np.float32)):
+            p = (ce[:3] - ce[3] * 1.01 + rng.random((5000, 3)) * 2.02 * ce[3]).astype(np.float32)
+            # points on the lower faces (zero numerators), one ulp either side,
+            # and on interior cell planes
+            lo = (ce[:3] - ce[3]).astype(np.float32)
+            edge = p[:300].copy()
+            edge[:100, 0] = lo[0]
+            edge[100:200, 1] = np.nextafter(lo[1], np.float32(np.inf))
+            edge[200:300, 2] = np.nextafter(lo[2], np.float32(-np.inf))
+            planes = (lo[None, :] + (2 * ce[3] / (lvoxc - 1)) * rng.integers(0, lvoxc, (300, 3))).astype(np.float32)
+            p = np.concatenate([p, edge, planes]).astype(np.float32)
+            got = pkg.probe.trilinear(brick, lvoxc, ce, p, leaf_rcp=leaf_rcp)
+            assert np.array_equal(got.view(np.uint32), oracle.voxel(brick, p, ce, lvoxc).view(np.uint32))
 
 
 @pytest.mark.parametrize("walk", ["octree", "grid"])

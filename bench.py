@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline wall time per build")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="target CPU-baseline wall time per build")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-steps", type=int, default=3)
@@ -205,22 +205,27 @@ def main():
         # Bounded sample: the full frame at a reduced number of paths per
         # pixel (sample indices 0..spp_cpu-1 of every pixel), sized from a
         # 1-spp probe to about --cpu-seconds of work, for each build of the
-        # restatement: the -O2 scalar one the tests use and an -O3
-        # x86-64-v3 build of the same code (identical numbers).
+        # restatement: one lane per path, and the reference's structure (2x2
+        # pixel packets traced in four lanes under masks, camera-ray leaves
+        # kept for all samples, secondary rays on a step traversal), each
+        # compiled -O2 and -O3 -march=x86-64-v3. All four give the same sums
+        # bit for bit.
         variants = []
         ref = None
-        for fast in (False, True):
-            tp = time.perf_counter()
-            O.render(wl, seed=args.seed, samples=1, threads=threads, fast=fast)
-            tp = time.perf_counter() - tp
-            spp_cpu = int(max(1, min(wl.spp, args.cpu_seconds / max(tp, 1e-3))))
-            tc = time.perf_counter()
-            r = O.render(wl, seed=args.seed, samples=spp_cpu, threads=threads, fast=fast)
-            tc = time.perf_counter() - tc
-            variants.append({"build": "-O3 -march=x86-64-v3" if fast else "-O2", "spp": spp_cpu,
-                             "seconds": round(tc, 2), "value": round(npix * spp_cpu / tc / 1e6, 4)})
-            if ref is None or spp_cpu < ref[1]:
-                ref = (r, spp_cpu)
+        for packets in (False, True):
+            for fast in (False, True):
+                tp = time.perf_counter()
+                O.render(wl, seed=args.seed, samples=1, threads=threads, fast=fast, packets=packets)
+                tp = time.perf_counter() - tp
+                spp_cpu = int(max(1, min(wl.spp, args.cpu_seconds / max(tp, 1e-3))))
+                tc = time.perf_counter()
+                r = O.render(wl, seed=args.seed, samples=spp_cpu, threads=threads, fast=fast, packets=packets)
+                tc = time.perf_counter() - tc
+                variants.append({"structure": "2x2 packets, 4 lanes" if packets else "one lane per path",
+                                 "build": "-O3 -march=x86-64-v3" if fast else "-O2", "spp": spp_cpu,
+                                 "seconds": round(tc, 2), "value": round(npix * spp_cpu / tc / 1e6, 4)})
+                if ref is None or spp_cpu < ref[1]:
+                    ref = (r, spp_cpu)
         best = max(variants, key=lambda v: v["value"])
         model = ""
         try:
@@ -229,9 +234,10 @@ def main():
         except OSError:
             pass
         cpu = {"value": best["value"], "unit": "Msamples/s", "cores": threads, "kind": "port",
-               "sample": "CPU oracle (oracle/oracle.c, one lane per path on %d threads), full %dx%d frame at %d of "
+               "sample": "CPU oracle (oracle/oracle.c, %s on %d threads, %s), full %dx%d frame at %d of "
                          "%d paths per pixel (%.1f s), same scene/camera/seed; fastest of the builds listed"
-                         % (threads, wl.w, wl.h, best["spp"], wl.spp, best["seconds"]),
+                         % (best["structure"], threads, best["build"], wl.w, wl.h, best["spp"], wl.spp,
+                            best["seconds"]),
                "builds": variants, "cpu_model": model, "nproc": os.cpu_count()}
         ref, spp_cpu = ref
         # Same sample on the GPU (same RNG keys) for a path-level parity check.

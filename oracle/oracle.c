@@ -5,8 +5,9 @@
  * compare the gfx950 kernels against, and the CPU baseline that bench.py
  * times beside them. The product path never links or calls it.
  *
- * Plain C, scalar, one lane per path, written from the reference's behaviour
- * (jaelpark/droplet-render):
+ * Plain C, one lane per path (or_render), written from the reference's
+ * behaviour (jaelpark/droplet-render); or_render_packets traces the same
+ * estimator in the reference's 2x2-packet, 4-lane structure (same sums):
  *   ray generation            src/kernel.cpp:447-487 (K_ParallelRender)
  *   estimator (recursive)     src/kernel.cpp:128-445 (SampleVolume), followed
  *                             structurally: the recursion, the masks per lane,
@@ -758,6 +759,481 @@ int or_render(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32
     if (threads > 256) threads = 256;
     pthread_t th[256];
     for (uint32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, render_worker, &J);
+    for (uint32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&J.mu);
+    return 0;
+}
+
+/* -------------------------------------------------------- packet path -- */
+/*
+ * The reference's CPU structure for the same estimator. K_ParallelRender
+ * (kernel.cpp:447-487) traces 2x2 pixel packets in 4-wide SSE lanes
+ * (BLCLOUD_VSIZE 4, BLCLOUD_VX = BLCLOUD_VY = 2, smmath.inl:4-6), and
+ * SampleVolume (kernel.cpp:128-445) runs the leaf loop, the Woodcock loop
+ * and both recursive calls for the whole packet in lockstep under lane
+ * masks: a lane that has collided, left its last leaf or is masked off
+ * idles until the packet's last lane is done (qm/sm/rm, kernel.cpp:206-316),
+ * and the continuation and shadow calls run for the packet with the mask of
+ * the lanes that scattered (gm1, kernel.cpp:393-406).
+ *
+ * Arithmetic that is the same for every lane (free-flight lengths, the
+ * positions, the xoshiro128** streams, the MIS combination) is 4-wide; the
+ * per-lane work (traversal lists, brick gathers, the Mie scan, libm
+ * transcendentals) is a loop over the active lanes, as the reference's
+ * gathers are (kernel.cpp:223-258, 287-310). A lane draws from its own
+ * stream only while it is active and rounds every operation as the
+ * one-lane restatement above does, so or_render_packets returns or_render's
+ * sums bit for bit; it is the CPU baseline in the reference's structure.
+ */
+typedef float v4f __attribute__((vector_size(16)));
+typedef int32_t v4i __attribute__((vector_size(16)));
+typedef uint32_t v4u __attribute__((vector_size(16)));
+
+static inline v4f vsel(v4i m, v4f a, v4f b) { return (v4f)(((v4i)a & m) | ((v4i)b & ~m)); }
+static inline v4i isel(v4i m, v4i a, v4i b) { return (a & m) | (b & ~m); }
+static inline v4u usel(v4i m, v4u a, v4u b) { return (a & (v4u)m) | (b & ~(v4u)m); }
+static inline int vany(v4i m) { return (m[0] | m[1] | m[2] | m[3]) != 0; }
+static inline v4f vsplat(float x) { return (v4f){x, x, x, x}; }
+
+typedef struct { v4u s[4]; } pstream;
+
+static void pstream_seed(pstream *r, const uint32_t key0[4], uint32_t key1, uint32_t sample, uint32_t ray) {
+    for (int j = 0; j < 4; ++j) {
+        ostream o;
+        stream_seed(&o, key0[j], key1, sample, ray);
+        for (int k = 0; k < 4; ++k) r->s[k][j] = o.s[k];
+    }
+}
+
+/* One xoshiro128** step (stream_next) of the lanes in m; the other lanes keep
+ * their state. Returns the uniforms of every lane (u01). */
+static v4f pstream_u(pstream *r, v4i m) {
+    v4u *s = r->s;
+    v4u x = s[1] * 5u;
+    v4u res = ((x << 7) | (x >> 25)) * 9u;
+    v4u t = s[1] << 9;
+    v4u n2 = s[2] ^ s[0], n3 = s[3] ^ s[1];
+    v4u n1 = s[1] ^ n2, n0 = s[0] ^ n3;
+    n2 ^= t;
+    n3 = (n3 << 11) | (n3 >> 21);
+    s[0] = usel(m, n0, s[0]);
+    s[1] = usel(m, n1, s[1]);
+    s[2] = usel(m, n2, s[2]);
+    s[3] = usel(m, n3, s[3]);
+    v4i q = (v4i)((res >> 8) | 1u);
+    return __builtin_convertvector(q, v4f) * vsplat(5.9604644775390625e-8f);
+}
+
+/* logf of the lanes in m (0 elsewhere). */
+static v4f plog(v4f u, v4i m) {
+    v4f r = vsplat(0.0f);
+    for (int j = 0; j < 4; ++j)
+        if (m[j]) r[j] = logf(u[j]);
+    return r;
+}
+
+/* OctreeStepTraverser (KernelOctree.cpp:222-349): the Revelles walk of
+ * process_subtree() with an explicit stack, one leaf per call, so a ray pays
+ * only for the leaves it reaches. Same leaves, same intervals, same order. */
+typedef struct { float t0[3], t1[3], tm[3]; uint32_t n, tn; } pframe;
+typedef struct {
+    pframe st[32];
+    int sp;                   /* frames on the stack */
+    uint32_t a;
+    int root_leaf;            /* the root itself is the only leaf */
+} pwalk;
+
+static void pwalk_push(const onode *ob, pwalk *W, const float t0[3], const float t1[3], uint32_t n) {
+    pframe *F = &W->st[W->sp++];
+    for (int k = 0; k < 3; ++k) {
+        F->t0[k] = t0[k];
+        F->t1[k] = t1[k];
+        F->tm[k] = 0.5f * (t0[k] + t1[k]);
+    }
+    F->n = n;
+    F->tn = first_node(F->t0, F->tm);
+    (void)ob;
+}
+
+/* OctreeInitialize (KernelOctree.cpp:78-110) + the root test. */
+static void pwalk_init(const onode *ob, pwalk *W, const float ro[3], const float rd[3]) {
+    const float *ce = ob[0].ce;
+    float o[3], d[3];
+    uint32_t a = 0;
+    for (int k = 0; k < 3; ++k) {
+        o[k] = (ro[k] - ce[k]) + ce[3];
+        d[k] = rd[k];
+    }
+    if (d[0] < 0.0f) { o[0] = 2.0f * ce[3] - o[0]; d[0] = -d[0]; a |= 4; }
+    if (d[1] < 0.0f) { o[1] = 2.0f * ce[3] - o[1]; d[1] = -d[1]; a |= 2; }
+    if (d[2] < 0.0f) { o[2] = 2.0f * ce[3] - o[2]; d[2] = -d[2]; a |= 1; }
+    float t0[3], t1[3], e2 = 2.0f * ce[3];
+    for (int k = 0; k < 3; ++k) {
+        t0[k] = (0.0f - o[k]) / d[k];
+        t1[k] = (e2 - o[k]) / d[k];
+    }
+    W->sp = 0;
+    W->a = a;
+    W->root_leaf = 0;
+    if (!(fmax3(t0) < fmin3(t1)) || t1[0] < 0.0f || t1[1] < 0.0f || t1[2] < 0.0f) return;
+    if (is_leaf(&ob[0])) {
+        W->root_leaf = 1;
+        memcpy(W->st[0].t0, t0, 12);
+        memcpy(W->st[0].t1, t1, 12);
+        return;
+    }
+    pwalk_push(ob, W, t0, t1, 0);
+}
+
+/* The next leaf of the walk: 1 with (node, entry, exit), or 0 at the end. */
+static int pwalk_next(const onode *ob, pwalk *W, uint32_t *node, float *ta, float *tb) {
+    static const uint32_t nla[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+    if (W->root_leaf) {
+        W->root_leaf = 0;
+        *node = 0;
+        *ta = fmax3(W->st[0].t0);
+        *tb = fmin3(W->st[0].t1);
+        return 1;
+    }
+    while (W->sp > 0) {
+        pframe *F = &W->st[W->sp - 1];
+        if (F->tn >= 8) {
+            W->sp--;
+            continue;
+        }
+        const float *t0 = F->t0, *t1 = F->t1, *tm = F->tm;
+        float a0[3], a1[3];
+        uint32_t tn = F->tn;
+        switch (tn) {
+        case 0:
+            memcpy(a0, t0, 12); memcpy(a1, tm, 12);
+            F->tn = next_node(tm, 4, 2, 1);
+            break;
+        case 1:
+            a0[0] = t0[0]; a0[1] = t0[1]; a0[2] = tm[2]; a1[0] = tm[0]; a1[1] = tm[1]; a1[2] = t1[2];
+            F->tn = next_node(a1, 5, 3, 8);
+            break;
+        case 2:
+            a0[0] = t0[0]; a0[1] = tm[1]; a0[2] = t0[2]; a1[0] = tm[0]; a1[1] = t1[1]; a1[2] = tm[2];
+            F->tn = next_node(a1, 6, 8, 3);
+            break;
+        case 3:
+            a0[0] = t0[0]; a0[1] = tm[1]; a0[2] = tm[2]; a1[0] = tm[0]; a1[1] = t1[1]; a1[2] = t1[2];
+            F->tn = next_node(a1, 7, 8, 8);
+            break;
+        case 4:
+            a0[0] = tm[0]; a0[1] = t0[1]; a0[2] = t0[2]; a1[0] = t1[0]; a1[1] = tm[1]; a1[2] = tm[2];
+            F->tn = next_node(a1, 8, 6, 5);
+            break;
+        case 5:
+            a0[0] = tm[0]; a0[1] = t0[1]; a0[2] = tm[2]; a1[0] = t1[0]; a1[1] = tm[1]; a1[2] = t1[2];
+            F->tn = next_node(a1, 8, 7, 8);
+            break;
+        case 6:
+            a0[0] = tm[0]; a0[1] = tm[1]; a0[2] = t0[2]; a1[0] = t1[0]; a1[1] = t1[1]; a1[2] = tm[2];
+            F->tn = next_node(a1, 8, 8, 7);
+            break;
+        default:
+            memcpy(a0, tm, 12); memcpy(a1, t1, 12);
+            F->tn = 8;
+            break;
+        }
+        uint32_t c = ob[F->n].chn[nla[tn ^ W->a]];
+        if (a1[0] < 0.0f || a1[1] < 0.0f || a1[2] < 0.0f || c == 0) continue;
+        if (is_leaf(&ob[c])) {
+            *node = c;
+            *ta = fmax3(a0);
+            *tb = fmin3(a1);
+            return 1;
+        }
+        if (W->sp == (int)(sizeof(W->st) / sizeof(W->st[0]))) abort();
+        pwalk_push(ob, W, a0, a1, c);
+    }
+    return 0;
+}
+
+typedef struct {
+    const oparams *P;
+    olist L[4];               /* the camera rays' leaves (full traverser, kept for every sample) */
+    pwalk W[4];               /* secondary rays (step traverser) */
+    uint32_t key0[4], sample;
+} pctx;
+
+/* The leaf and Woodcock loops (kernel.cpp:197-317) for the lanes in gm;
+ * track() for each lane. Returns the mask of the lanes that collided. */
+static v4i ptrack(pctx *X, const v4f ro[3], const v4f rd[3], v4i gm, uint32_t ray_id, v4f depth, v4f *ptd) {
+    const oparams *P = X->P;
+    pstream rng;
+    pstream_seed(&rng, X->key0, P->key1, X->sample, ray_id);
+    /* Camera rays walk the leaf lists made once per packet; the others step
+     * their traversal along, every lane that still has leaves advancing with
+     * each round of the leaf loop (GetLeaf, kernel.cpp:206-218). */
+    const int primary = ray_id == 0u;
+    v4i lm = gm;
+    if (!primary)
+        for (int j = 0; j < 4; ++j)
+            if (gm[j]) {
+                float o[3] = {ro[0][j], ro[1][j], ro[2][j]}, d[3] = {rd[0][j], rd[1][j], rd[2][j]};
+                pwalk_init(P->ob, &X->W[j], o, d);
+            }
+    const v4f sige = vsplat(P->sigma_a + P->sigma_s);
+    v4f td = vsplat(0.0f);
+    v4i rm = {-1, -1, -1, -1};
+    for (uint32_t i = 0;; ++i) {
+        uint32_t lnode[4] = {0, 0, 0, 0};
+        float lt0[4] = {0, 0, 0, 0}, lt1[4] = {0, 0, 0, 0};
+        for (int j = 0; j < 4; ++j) {
+            if (!lm[j]) continue;
+            if (primary) {
+                if (i < X->L[j].n) {
+                    lnode[j] = X->L[j].ls[i].node;
+                    lt0[j] = X->L[j].ls[i].t0;
+                    lt1[j] = X->L[j].ls[i].t1;
+                } else {
+                    lm[j] = 0;
+                }
+            } else if (!pwalk_next(P->ob, &X->W[j], &lnode[j], &lt0[j], &lt1[j])) {
+                lm[j] = 0;
+            }
+        }
+        v4i qm = gm & rm & lm & (td < depth);
+        if (!vany(qm)) break;
+        v4f tra = vsplat(0.0f), trb = vsplat(0.0f), smax = vsplat(1.0f);
+        v4i skip = {0, 0, 0, 0};
+        const onode *N[4] = {NULL, NULL, NULL, NULL};
+        const float *fogb[4] = {NULL, NULL, NULL, NULL}, *sdfb[4] = {NULL, NULL, NULL, NULL};
+        for (int j = 0; j < 4; ++j) {
+            if (!qm[j]) continue;
+            const onode *Nj = &P->ob[lnode[j]];
+            N[j] = Nj;
+            tra[j] = lt0[j];
+            trb[j] = lt1[j];
+            const int sdf = Nj->volx[0] != ONO;
+            fogb[j] = Nj->volx[1] != ONO ? P->pvol_fog + (size_t)Nj->volx[1] * P->lvoxc3 : NULL;
+            sdfb[j] = sdf ? P->pvol_sdf + (size_t)Nj->volx[0] * P->lvoxc3 : NULL;
+            smax[j] = sdf ? 1.0f : Nj->qval[1];
+            int sk = 1;
+            if (sdf) {
+                sk = 0;
+                if (tra[j] > td[j]) {
+                    float r0[3] = {ro[0][j] + rd[0][j] * tra[j], ro[1][j] + rd[1][j] * tra[j],
+                                   ro[2][j] + rd[2][j] * tra[j]};
+                    sk = or_voxel(sdfb[j], r0, Nj->ce, P->lvoxc) > 0.0f;
+                }
+            }
+            skip[j] = sk ? -1 : 0;
+        }
+        const v4f lo = td;
+        const v4f tr0 = tra - td, tr1 = trb - td;
+        skip &= qm;
+        td = vsel(skip, tra, td);
+        const v4f s0 = vsel(skip, tr0, vsplat(0.0f));
+        const v4f den = sige * smax;
+        v4i sm = qm;
+        v4f sr = -(plog(pstream_u(&rng, sm), sm) / den);
+        for (;;) {
+            sm &= rm;
+            if (!vany(sm)) break;
+            v4f sc = s0 + sr;
+            td = vsel(sm, lo + sc, td);
+            v4i inb = sc < tr1;
+            td = vsel(sm & ~inb, trb, td);
+            sm &= inb;
+            rm = isel(sm, sc > tr0, rm);
+            v4i act = sm & rm;
+            if (!vany(act)) continue;
+            v4f rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
+            v4f p = vsplat(-1.0f), d = vsplat(1.0f);
+            for (int j = 0; j < 4; ++j) {
+                if (!act[j]) continue;
+                float x[3] = {rc[0][j], rc[1][j], rc[2][j]};
+                if (sdfb[j]) {
+                    d[j] = or_voxel(sdfb[j], x, N[j]->ce, P->lvoxc);
+                    p[j] = (d[j] > 0.0f && fogb[j]) ? or_voxel(fogb[j], x, N[j]->ce, P->lvoxc) : -1.0f;
+                } else {
+                    p[j] = or_voxel(fogb[j], x, N[j]->ce, P->lvoxc);
+                }
+            }
+            v4f q = pstream_u(&rng, act);
+            rm = isel(act, (p / smax < q) & (d > 0.0f), rm);
+            v4i cont = act & rm;
+            if (vany(cont)) sr = vsel(cont, sr - plog(pstream_u(&rng, cont), cont) / den, sr);
+        }
+    }
+    *ptd = td;
+    return gm & ~rm;
+}
+
+/* SampleVolume (kernel.cpp:128-445) for a packet with lane mask gm, one
+ * sample, depth r; sample_volume() for each lane of gm. */
+static void psample_volume(pctx *X, const v4f ro[3], const v4f rd[3], v4i gm, uint32_t r, uint32_t ray_id,
+                           v4f depth, v4f cl[4], v4f cs[4]) {
+    const oparams *P = X->P;
+    const v4f zero = vsplat(0.0f), one = vsplat(1.0f);
+    v4f td;
+    const v4i scat = ptrack(X, ro, rd, gm, ray_id, depth, &td);
+    const v4i rm = ~scat;
+    v4f lc[4] = {zero, zero, zero, zero}, le[4] = {zero, zero, zero, zero};
+    const v4i esc = gm & rm;
+    if (r > 0) {
+        for (int j = 0; j < 4; ++j) {
+            if (!esc[j]) continue;
+            float d[3] = {rd[0][j], rd[1][j], rd[2][j]}, l3[3] = {0.0f, 0.0f, 0.0f}, e3[3];
+            for (uint32_t l = 0; l < P->light_count; ++l)
+                if (dot3(d, P->lights[l].dir) > P->lights[l].cosang)
+                    for (int k = 0; k < 3; ++k) l3[k] += P->lights[l].color[k];
+            env_eval(P, d, e3);
+            for (int k = 0; k < 3; ++k) {
+                lc[k][j] = l3[k];
+                le[k][j] = e3[k];
+            }
+        }
+        lc[3] = le[3] = one;
+    } else {
+        lc[3] = le[3] = vsel(rm, zero, one);
+    }
+    if (r < P->scattevs && vany(scat)) {
+        v4f srd[3] = {zero, zero, one}, lrd[3] = {zero, zero, one}, p1[3] = {one, one, one}, p3[3] = {one, one, one};
+        v4f p2 = zero, pd1 = zero;
+        for (int j = 0; j < 4; ++j) {
+            if (!scat[j]) continue;
+            uint32_t c[4] = {X->sample, r, 0u, 1u}, v[4];
+            philox(c, X->key0[j], P->key1, v);
+            float u1 = u01(v[0]), u2 = u01(v[1]), u3 = u01(v[2]), u4 = u01(v[3]);
+            float d[3] = {rd[0][j], rd[1][j], rd[2][j]}, s[3], l[3], q1[3], q3[3];
+            if (P->phase == 0) or_mie_sample(P->mie, d, u1, u2, s);
+            else or_hg_sample(P->hg_g, d, u1, u2, s);
+            sun_sample(&P->lights[0], u3, u4, l);
+            if (P->phase == 0) {
+                or_mie_eval(P->mie, dot3(s, d), q1);
+                or_mie_eval(P->mie, dot3(l, d), q3);
+            } else {
+                float h1 = or_hg_eval(P->hg_g, dot3(s, d)), h3 = or_hg_eval(P->hg_g, dot3(l, d));
+                q1[0] = q1[1] = q1[2] = h1;
+                q3[0] = q3[1] = q3[2] = h3;
+            }
+            p2[j] = sun_pdf(&P->lights[0], l);
+            pd1[j] = sun_pdf(&P->lights[0], s);
+            for (int k = 0; k < 3; ++k) {
+                srd[k][j] = s[k];
+                lrd[k][j] = l[k];
+                p1[k][j] = q1[k];
+                p3[k][j] = q3[k];
+            }
+        }
+        v4f rc[3] = {ro[0] + rd[0] * td, ro[1] + rd[1] * td, ro[2] + rd[2] * td};
+        v4f S1l[4], S1s[4], S2l[4], S2s[4];
+        psample_volume(X, rc, srd, scat, r + 1, 2u * r + 2u, vsplat(FLT_MAX), S1l, S1s);
+        psample_volume(X, rc, lrd, scat, P->scattevs, 2u * r + 1u, vsplat(FLT_MAX), S2l, S2s);
+        const v4f sigs = vsplat(P->sigma_s), sige = vsplat(P->sigma_a + P->sigma_s);
+        for (int k = 0; k < 3; ++k) {
+            v4f w1 = p1[k] / (p1[k] + pd1);
+            v4f w2 = p3[k] / (p3[k] + p2);
+            v4f cl1 = ((S1l[k] * w1 + S2l[k] * w2) * sigs) / sige;
+            v4f cs1 = ((S1s[k] * w1 + zero * w2) * sigs) / sige;
+            cl[k] = zero + vsel(scat, cl1, vsel(rm, lc[k], zero));
+            cs[k] = zero + vsel(scat, cs1, vsel(rm, le[k], zero));
+        }
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            cl[k] = zero + vsel(rm, lc[k], zero);
+            cs[k] = zero + vsel(rm, le[k], zero);
+        }
+    }
+    cl[3] = zero + lc[3];
+    cs[3] = zero + le[3];
+}
+
+/* One 2x2 packet (K_ParallelRender, kernel.cpp:453-476): lanes (0,0), (1,0),
+ * (0,1), (1,1) of the packet at tile-local pixel (2*qx, 2*qy); lanes past the
+ * tile's rendered extent are masked off like those outside the image. */
+static void render_packet(ojob *J, pctx *X, uint32_t qx, uint32_t qy) {
+    const oparams *P = J->P;
+    v4f ro[3], rd[3], depth;
+    v4i gm;
+    size_t o[4];
+    for (int j = 0; j < 4; ++j) {
+        uint32_t lx = 2u * qx + (uint32_t)(j & 1), ly = 2u * qy + (uint32_t)(j >> 1);
+        uint32_t px = J->x0 + lx, py = J->y0 + ly;
+        float a[3], b[3];
+        gm[j] = camera_ray(P, px, py, a, b) && lx < J->rx && ly < J->ry ? -1 : 0;
+        for (int k = 0; k < 3; ++k) {
+            ro[k][j] = a[k];
+            rd[k][j] = b[k];
+        }
+        X->key0[j] = py * P->w + px;
+        depth[j] = P->depth ? P->depth[X->key0[j]] : FLT_MAX;
+        o[j] = ((size_t)ly * J->tilex + lx) * 4;
+    }
+    if (!vany(gm)) return;
+    /* The camera rays' leaves, once for all samples (the reference's caching
+     * full traverser, kernel.cpp:130-134). */
+    for (int j = 0; j < 4; ++j) {
+        X->L[j].n = 0;
+        if (gm[j]) {
+            float a[3] = {ro[0][j], ro[1][j], ro[2][j]}, b[3] = {rd[0][j], rd[1][j], rd[2][j]};
+            traverse(&X->L[j], a, b);
+        }
+    }
+    double acc[4][4][4] = {{{0}}};
+    for (uint32_t s = 0; s < J->samples; ++s) {
+        X->sample = J->sample_base + s;
+        v4f cl[4], cs[4];
+        psample_volume(X, ro, rd, gm, 0, 0u, depth, cl, cs);
+        for (int j = 0; j < 4; ++j)
+            for (int k = 0; k < 4; ++k) {
+                acc[j][0][k] += cl[k][j];
+                acc[j][1][k] += cs[k][j];
+                acc[j][2][k] += k < 3 ? (double)cl[k][j] * cl[k][j] : cl[k][j];
+                acc[j][3][k] += k < 3 ? (double)cs[k][j] * cs[k][j] : cs[k][j];
+            }
+    }
+    for (int j = 0; j < 4; ++j) {
+        if (!gm[j]) continue;
+        for (int b = 0; b < 4; ++b)
+            for (int k = 0; k < 4; ++k) J->out[b * J->plane + o[j] + k] = (float)acc[j][b][k];
+    }
+}
+
+static void *render_packet_worker(void *arg) {
+    ojob *J = (ojob *)arg;
+    pctx X;
+    memset(&X, 0, sizeof(X));
+    X.P = J->P;
+    for (int j = 0; j < 4; ++j) X.L[j].ob = J->P->ob;
+    for (;;) {
+        pthread_mutex_lock(&J->mu);
+        uint32_t row = J->next_row++;
+        pthread_mutex_unlock(&J->mu);
+        if (2u * row >= J->ry) break;
+        for (uint32_t qx = 0; 2u * qx < J->rx; ++qx) render_packet(J, &X, qx, row);
+    }
+    for (int j = 0; j < 4; ++j) free(X.L[j].ls);
+    return NULL;
+}
+
+/* or_render with the reference's 2x2-packet, 4-lane structure: same
+ * arguments, same output bits; threads take rows of packets. */
+int or_render_packets(const oparams *P, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
+                      uint32_t sample_base, uint32_t threads, float *out) {
+    ojob J;
+    memset(&J, 0, sizeof(J));
+    J.P = P;
+    J.x0 = x0; J.y0 = y0;
+    J.rx = tilex & ~1u; J.ry = tiley & ~1u;
+    if (x0 + J.rx > P->w) J.rx = x0 < P->w ? P->w - x0 : 0;
+    if (y0 + J.ry > P->h) J.ry = y0 < P->h ? P->h - y0 : 0;
+    J.tilex = tilex;
+    J.samples = samples;
+    J.sample_base = sample_base;
+    J.out = out;
+    J.plane = (size_t)tilex * tiley * 4;
+    memset(out, 0, sizeof(float) * 4 * J.plane);
+    pthread_mutex_init(&J.mu, NULL);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    for (uint32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, render_packet_worker, &J);
     for (uint32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
     pthread_mutex_destroy(&J.mu);
     return 0;

@@ -258,17 +258,20 @@ def make_params(wl, seed, depth=None):
 
 
 def render(wl, seed, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, samples=None, threads=None, depth=None,
-           fast=False):
+           fast=False, packets=False):
     """CPU render of a tile; returns [lights, env, sumsq lights, sumsq env] as (tiley, tilex, 4) float32.
-    fast: the -O3 x86-64-v3 build of the same restatement."""
+    fast: the -O3 x86-64-v3 build of the same restatement.
+    packets: the reference's structure (2x2 pixel packets traced in 4 lanes
+    under masks, or_render_packets); the same sums bit for bit."""
     tilex = tilex or wl.w
     tiley = tiley or wl.h
     samples = samples or wl.spp
     threads = threads or os.cpu_count() or 1
     P, keep = make_params(wl, seed, depth)
     out = np.zeros((4, tiley, tilex, 4), np.float32)
-    lib(fast).or_render(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
-                        C.c_uint32(samples), C.c_uint32(sample_base), C.c_uint32(threads), fp(out))
+    fn = lib(fast).or_render_packets if packets else lib(fast).or_render
+    fn(C.byref(P), C.c_uint32(x0), C.c_uint32(y0), C.c_uint32(tilex), C.c_uint32(tiley),
+       C.c_uint32(samples), C.c_uint32(sample_base), C.c_uint32(threads), fp(out))
     del keep
     return [out[0], out[1], out[2], out[3]]
 

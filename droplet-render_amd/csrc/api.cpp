@@ -111,7 +111,7 @@ struct drk_kernel {
     size_t partial_bytes = 0;
     void *d_mail = nullptr;         // escape-service mailboxes, two float4 per lane of the grid
     size_t mail_bytes = 0;
-    uint32_t *d_counter = nullptr;
+    uint32_t *d_counter = nullptr;  // launch block: counter block, then the launch's parameters
     unsigned long long *d_counts = nullptr;
     float *d_env = nullptr;
     KernelConsts C;                 // host copy of the per-kernel constants
@@ -525,7 +525,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     hipError_t e = hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking);
     size_t tile_bytes = (size_t)tilex * tiley * 16;
     if (e == hipSuccess) e = dev_malloc(&k->d_out, 4 * tile_bytes);
-    if (e == hipSuccess) e = dev_malloc(&k->d_counter, kCounterWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = dev_malloc(&k->d_counter, kLaunchBlockBytes);
     if (e == hipSuccess) e = dev_malloc(&k->d_counts, 64);
     if (e == hipSuccess && smp->env == DRK_ENV_MAP) {
         size_t eb = (size_t)smp->env_w * smp->env_h * 16;

@@ -1137,29 +1137,30 @@ template <int kMode>
 // Four waves per SIMD (at most 128 VGPRs; the root interval and the
 // continuation state wait in LDS so that nothing spills): the fourth wave
 // hides more of the traversal's dependent loads (C1: 52.9 -> 46.9 ms).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render(RenderParams P, WorkParams W) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render(
+    const LaunchParams *__restrict__ L) {
     __shared__ RenderLds S;
     render_lds_init(S);
-    render_body<kMode, true>(P, W, S);
+    render_body<kMode, true>(L->P, L->W, S);
 }
 
 // The adjoint, like the forward, at four waves per SIMD: its image gradient
 // is re-read from the tiles where needed, not kept in registers, so it fits
 // 128 VGPRs without spills (C1 forward+adjoint step 132.8 -> 124.3 ms).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint(RenderParams P,
-                                                                                                    WorkParams W) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint(
+    const LaunchParams *__restrict__ L) {
     __shared__ RenderLds S;
     render_lds_init(S);
-    render_body<kModeAdjoint, false>(P, W, S);
+    render_body<kModeAdjoint, false>(L->P, L->W, S);
 }
 
 // The adjoint with null-collision terms (the pathwise derivative of the
 // weighted forward), same occupancy as the default adjoint.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint_null(RenderParams P,
-                                                                                                         WorkParams W) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint_null(
+    const LaunchParams *__restrict__ L) {
     __shared__ RenderLds S;
     render_lds_init(S);
-    render_body<kModeAdjointNull, false>(P, W, S);
+    render_body<kModeAdjointNull, false>(L->P, L->W, S);
 }
 
 // Many launches (scenes, views, tiles) in one persistent grid. Workgroup b
@@ -1207,6 +1208,17 @@ static void launch_reduce_kernel(const WorkParams &W, uint32_t rx, uint32_t tile
     const uint32_t nb = (W.npix + 255) / 256;
     if (buffers == 4) k_reduce<4><<<nb, 256, 0, s>>>(W, rx, tilex);
     else k_reduce<2><<<nb, 256, 0, s>>>(W, rx, tilex);
+}
+
+// Launch setup: clears the counter block and stores the launch's parameters
+// after it (one workgroup).
+__global__ void k_launch_setup(RenderParams P, WorkParams W) {
+    for (uint32_t i = threadIdx.x; i < kCounterWords; i += blockDim.x) W.counter[i] = 0u;
+    if (threadIdx.x == 0) {
+        LaunchParams *L = reinterpret_cast<LaunchParams *>(reinterpret_cast<char *>(W.counter) + kParamOffset);
+        L->P = P;
+        L->W = W;
+    }
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -1269,16 +1281,17 @@ hipError_t launch_reduce(const WorkParams &W, uint32_t rx, uint32_t tilex, uint3
 
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1) {
-    k_fill_u32<<<(kCounterWords + 255) / 256, 256, 0, s>>>(W.counter, kCounterWords, 0u);
+    k_launch_setup<<<1, 256, 0, s>>>(P, W);
+    const LaunchParams *L = reinterpret_cast<const LaunchParams *>(reinterpret_cast<char *>(W.counter) + kParamOffset);
     if (ev0) (void)hipEventRecord(ev0, s);
     switch (mode) {
-    case kModeForward: k_render<kModeForward><<<grid, kBlock, 0, s>>>(P, W); break;
-    case kModeForwardSq: k_render<kModeForwardSq><<<grid, kBlock, 0, s>>>(P, W); break;
-    case kModeCount: k_render<kModeCount><<<grid, kBlock, 0, s>>>(P, W); break;
-    case kModeRecord: k_render<kModeRecord><<<grid, kBlock, 0, s>>>(P, W); break;
-    case kModeWeighted: k_render<kModeWeighted><<<grid, kBlock, 0, s>>>(P, W); break;
-    case kModeAdjointNull: k_render_adjoint_null<<<grid, kBlock, 0, s>>>(P, W); break;
-    default: k_render_adjoint<<<grid, kBlock, 0, s>>>(P, W); break;
+    case kModeForward: k_render<kModeForward><<<grid, kBlock, 0, s>>>(L); break;
+    case kModeForwardSq: k_render<kModeForwardSq><<<grid, kBlock, 0, s>>>(L); break;
+    case kModeCount: k_render<kModeCount><<<grid, kBlock, 0, s>>>(L); break;
+    case kModeRecord: k_render<kModeRecord><<<grid, kBlock, 0, s>>>(L); break;
+    case kModeWeighted: k_render<kModeWeighted><<<grid, kBlock, 0, s>>>(L); break;
+    case kModeAdjointNull: k_render_adjoint_null<<<grid, kBlock, 0, s>>>(L); break;
+    default: k_render_adjoint<<<grid, kBlock, 0, s>>>(L); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1617,8 +1630,8 @@ hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Fixed scratch for the allocation-free render entry point: the launch's
-// counter block and its per-chunk partial sums live in this file-scope
-// device array instead of in buffers the library allocates. Every GPU the
+// counter block and parameters and its per-chunk partial sums live in this
+// file-scope device array instead of in buffers the library allocates. Every GPU the
 // library runs on holds its own copy (kScratchBytes) until the process ends.
 // In host code the array's name is not a device address: scratch_address
 // looks the address up once per device with hipGetSymbolAddress and the

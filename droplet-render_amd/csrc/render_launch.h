@@ -145,21 +145,38 @@ inline void set_bands(WorkParams &W, uint32_t rx) {
     W.rx_div = udiv_make(rx ? rx : 1u);
 }
 
+// A launch's parameters in device memory, written by a setup kernel right
+// before the launch (so the sequence stays kernels only and can be captured
+// in a graph): the render kernels read them through a pointer with scalar
+// loads where they are used. Passed by value, the argument block (148 words)
+// was kept in scalar registers across the state machine's loop, and the
+// registers it did not fit spilled into vector-register lanes whose
+// read-backs took vector issue slots (C1 forward 27.7 -> 26.7 ms).
+struct LaunchParams {
+    RenderParams P;
+    WorkParams W;
+};
+// The launch block: the counter block, then the parameters at kParamOffset.
+constexpr size_t kParamOffset = 1280;
+constexpr size_t kLaunchBlockBytes = kParamOffset + sizeof(LaunchParams);
+static_assert(kCounterWords * sizeof(uint32_t) <= kParamOffset, "counter block fits before the parameters");
+
 int render_block_size();
 int render_blocks_per_cu(int mode);
-// ev0/ev1 (may be null) are recorded on s right before and after k_render.
+// W.counter is the launch block (kLaunchBlockBytes). ev0/ev1 (may be null)
+// are recorded on s right before and after k_render.
 hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t grid, int mode, hipStream_t s,
                          hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 // Fixed device scratch of the allocation-free render (one per device, the
-// size below): the counter block at offset 0, partial sums from
-// kScratchPartialOffset. scratch_address gives the current device's copy
+// size below): the launch block (counters and parameters) at offset 0,
+// partial sums from kScratchPartialOffset. scratch_address gives the current device's copy
 // (looked up once per device).
 constexpr size_t kScratchBytes = 128ull << 20;
 constexpr size_t kScratchPartialOffset = 4096;
 constexpr uint32_t kScratchDevices = 64;
 hipError_t scratch_address(void **ptr);
-static_assert(kCounterWords * sizeof(uint32_t) <= kScratchPartialOffset, "counter block fits before the partials");
+static_assert(kLaunchBlockBytes <= kScratchPartialOffset, "launch block fits before the partials");
 // Batched forward launch over njobs (P, W) pairs in device memory; counters
 // (ncounters words, every job's band counters) are cleared first.
 int render_batch_blocks_per_cu(int mode);

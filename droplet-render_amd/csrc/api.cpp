@@ -111,6 +111,8 @@ struct drk_kernel {
     size_t partial_bytes = 0;
     void *d_mail = nullptr;         // escape-service mailboxes, two float4 per lane of the grid
     size_t mail_bytes = 0;
+    void *d_cam = nullptr;          // camera rays of the rendered extent, one float4 per pixel
+    size_t cam_bytes = 0;
     uint32_t *d_counter = nullptr;  // launch block: counter block, then the launch's parameters
     unsigned long long *d_counts = nullptr;
     float *d_env = nullptr;
@@ -818,6 +820,21 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         }
         W.esc_mail = static_cast<float4 *>(k->d_mail);
     }
+    // Camera rays: every sample of a pixel reuses its one primary ray
+    // (src/kernel.cpp:556), so a pass before the render computes them once
+    // per pixel instead of once per path (the allocation-free entry computes
+    // them per path).
+    if (!noalloc && npix > 0) {
+        const size_t need = (size_t)npix * 16;
+        if (need > k->cam_bytes) {
+            if (k->d_cam) (void)hipFree(k->d_cam);
+            k->d_cam = nullptr;
+            k->cam_bytes = 0;
+            HIP_TRY(dev_malloc(&k->d_cam, need), "render: camera rays");
+            k->cam_bytes = need;
+        }
+        W.cam = static_cast<float4 *>(k->d_cam);
+    }
     if (count) HIP_TRY(hipMemsetAsync(k->d_counts, 0, 16, s), "render: counters");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (k->timing && !count && !noalloc) {
@@ -1123,6 +1140,7 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_out) (void)hipFree(k->d_out);
     if (k->d_partial) (void)hipFree(k->d_partial);
     if (k->d_mail) (void)hipFree(k->d_mail);
+    if (k->d_cam) (void)hipFree(k->d_cam);
     if (k->d_counter) (void)hipFree(k->d_counter);
     if (k->d_counts) (void)hipFree(k->d_counts);
     if (k->d_env) (void)hipFree(k->d_env);

@@ -361,7 +361,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 
     auto start_path = [&]() {
         bool gm;
-        camera_ray(K, P.w, P.h, key0 % P.w, key0 / P.w, ro, rd, gm);
+        if (W.cam) {
+            const float4 c = W.cam[pix_local];
+            ro = mk3(K.viewi[12], K.viewi[13], K.viewi[14]);
+            rd = mk3(c.x, c.y, c.z);
+            gm = c.w != 0.0f;
+        } else {
+            camera_ray(K, P.w, P.h, key0 % P.w, key0 / P.w, ro, rd, gm);
+        }
         beta = mk3(1, 1, 1);
         pathL = mk3(0, 0, 0);
         pathE = mk3(0, 0, 0);
@@ -1221,6 +1228,19 @@ __global__ void k_launch_setup(RenderParams P, WorkParams W) {
     }
 }
 
+// The camera ray of every pixel of the rendered extent (W.cam).
+__global__ void k_camera_rays(const LaunchParams *__restrict__ L) {
+    const RenderParams &P = L->P;
+    const WorkParams &W = L->W;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W.npix) return;
+    const uint32_t ly = udiv(i, W.rx_div), lx = i - ly * P.rx;
+    F3 ro, rd;
+    bool gm;
+    camera_ray(*P.kc, P.w, P.h, P.x0 + lx, P.y0 + ly, ro, rd, gm);
+    W.cam[i] = make_float4(rd.x, rd.y, rd.z, gm ? 1.0f : 0.0f);
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -1283,6 +1303,7 @@ hipError_t launch_render(const RenderParams &P, const WorkParams &W, uint32_t gr
                          hipEvent_t ev0, hipEvent_t ev1) {
     k_launch_setup<<<1, 256, 0, s>>>(P, W);
     const LaunchParams *L = reinterpret_cast<const LaunchParams *>(reinterpret_cast<char *>(W.counter) + kParamOffset);
+    if (W.cam && W.npix) k_camera_rays<<<(W.npix + 255) / 256, 256, 0, s>>>(L);
     if (ev0) (void)hipEventRecord(ev0, s);
     switch (mode) {
     case kModeForward: k_render<kModeForward><<<grid, kBlock, 0, s>>>(L); break;

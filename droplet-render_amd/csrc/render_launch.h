@@ -94,6 +94,9 @@ struct WorkParams {
     uint32_t nchunks;
     float *partial;           // [nchunks][npix][buffers] float4 when nchunks > 1, else nullptr: a unit's
                               // buffers side by side, written by one lane as one 32/64-byte span
+    float4 *cam;              // per pixel of the rendered extent (pixel index as for units): the camera
+                              // ray's direction and lane mask (w: 1 inside the image), written by a
+                              // pass before the render; nullptr: each path computes its camera ray
     float4 *esc_mail;         // per lane of the grid two float4: the escape service's mailbox (nullptr:
                               // escape radiance computed in place)
     float *out_l, *out_e;     // tile float4 outputs (stride tilex)

@@ -354,57 +354,69 @@ __device__ __forceinline__ void grid_to_stack(const uint32_t *__restrict__ chx, 
     }
 }
 
-// One call of the grid walk, with the contract of trav_next: at most
-// `budget` cell steps, then kTravPending; a hand-over continues in
-// trav_next within the same budget.
+// One cell of the grid walk: kTravLeaf (its leaf and [tra, trb]),
+// kTravExhausted, kTravPending (an empty cell was stepped over), or
+// kTravRevelles (a tie: the Revelles stack has been rebuilt for the current
+// position and the walk continues in trav_next). One exit per outcome and no
+// loop, so that the walk's state is not shuffled between registers at the
+// merges of an inlined loop.
+constexpr int kTravRevelles = 3;
+__device__ __forceinline__ int grid_step(const uint32_t *__restrict__ cells, uint32_t L,
+                                         const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk,
+                                         const float *root, uint32_t stride, uint32_t &leaf, float &tra, float &trb,
+                                         uint32_t &touches) {
+    const uint32_t n = 1u << L;
+    uint32_t packed = stk[0];
+    if (packed & kGridPending) {
+        packed &= ~kGridPending;
+    } else {
+        // next_node's choice of the exit plane (ties as the reference breaks them)
+        const bool lxy = T.t1.x < T.t1.y, lxz = T.t1.x < T.t1.z, lyz = T.t1.y < T.t1.z;
+        const uint32_t ax = lxy ? (lxz ? 0u : 2u) : (lyz ? 1u : 2u);
+        const float m = ax == 0 ? T.t1.x : (ax == 1 ? T.t1.y : T.t1.z);
+        const bool tie = (ax != 0 && T.t0.x >= m) || (ax != 1 && T.t0.y >= m) || (ax != 2 && T.t0.z >= m);
+        if (tie) {
+            grid_to_stack(chx, T, stk, root, stride, L, packed);
+            return kTravRevelles;
+        }
+        const uint32_t id = (packed >> (8 * ax)) & 0xffu;
+        if (id + 1 == n) {
+            T.lvl = -1;
+            return kTravExhausted;
+        }
+        packed += 1u << (8 * ax);
+        // the root interval of that axis (see trav_set_root's layout)
+        const float lo = root[ax * stride], hi = root[(3 + ax) * stride];
+        const float nx = grid_boundary(lo, hi, L, id + 2);
+        if (ax == 0) { T.t0.x = m; T.t1.x = nx; }
+        else if (ax == 1) { T.t0.y = m; T.t1.y = nx; }
+        else { T.t0.z = m; T.t1.z = nx; }
+    }
+    // the cell's leaf, if any (grid indices are in the unmirrored frame)
+    const uint32_t ix = packed & 0xffu, iy = (packed >> 8) & 0xffu, iz = (packed >> 16) & 0xffu;
+    const uint32_t ux = (T.a & 4) ? n - 1 - ix : ix, uy = (T.a & 2) ? n - 1 - iy : iy, uz = (T.a & 1) ? n - 1 - iz : iz;
+    const uint32_t cell = cells[(uz * n + uy) * n + ux];
+    touches += 1;
+    stk[0] = packed;
+    if (!cell) return kTravPending;
+    leaf = cell;
+    tra = fmaxf(fmaxf(T.t0.x, T.t0.y), T.t0.z);
+    trb = fminf(fminf(T.t1.x, T.t1.y), T.t1.z);
+    return kTravLeaf;
+}
+
+// The grid walk with the contract of trav_next: at most `budget` cell
+// steps, then kTravPending; a hand-over continues in trav_next within the
+// same budget.
 __device__ __forceinline__ int grid_next(const uint32_t *__restrict__ cells, uint32_t L,
                                          const uint32_t *__restrict__ chx, Traversal &T, uint32_t *stk,
                                          const float *root, uint32_t stride, uint32_t &leaf, float &tra, float &trb,
                                          uint32_t &touches, uint32_t budget) {
-    const uint32_t n = 1u << L;
-    uint32_t packed = stk[0];
     for (uint32_t steps = 0;; ++steps) {
-        if (steps >= budget) {
-            stk[0] = packed;
-            return kTravPending;
-        }
-        if (packed & kGridPending) {
-            packed &= ~kGridPending;
-        } else {
-            // next_node's choice of the exit plane (ties as the reference breaks them)
-            const bool lxy = T.t1.x < T.t1.y, lxz = T.t1.x < T.t1.z, lyz = T.t1.y < T.t1.z;
-            const uint32_t ax = lxy ? (lxz ? 0u : 2u) : (lyz ? 1u : 2u);
-            const float m = ax == 0 ? T.t1.x : (ax == 1 ? T.t1.y : T.t1.z);
-            const bool tie = (ax != 0 && T.t0.x >= m) || (ax != 1 && T.t0.y >= m) || (ax != 2 && T.t0.z >= m);
-            if (tie) {
-                grid_to_stack(chx, T, stk, root, stride, L, packed);
-                return trav_next(chx, T, stk, root, stride, leaf, tra, trb, touches, budget - steps);
-            }
-            const uint32_t id = (packed >> (8 * ax)) & 0xffu;
-            if (id + 1 == n) {
-                T.lvl = -1;
-                return kTravExhausted;
-            }
-            packed += 1u << (8 * ax);
-            // the root interval of that axis (see trav_set_root's layout)
-            const float lo = root[ax * stride], hi = root[(3 + ax) * stride];
-            const float nx = grid_boundary(lo, hi, L, id + 2);
-            if (ax == 0) { T.t0.x = m; T.t1.x = nx; }
-            else if (ax == 1) { T.t0.y = m; T.t1.y = nx; }
-            else { T.t0.z = m; T.t1.z = nx; }
-        }
-        // the cell's leaf, if any (grid indices are in the unmirrored frame)
-        const uint32_t ix = packed & 0xffu, iy = (packed >> 8) & 0xffu, iz = (packed >> 16) & 0xffu;
-        const uint32_t ux = (T.a & 4) ? n - 1 - ix : ix, uy = (T.a & 2) ? n - 1 - iy : iy, uz = (T.a & 1) ? n - 1 - iz : iz;
-        const uint32_t cell = cells[(uz * n + uy) * n + ux];
-        touches += 1;
-        if (cell) {
-            leaf = cell;
-            tra = fmaxf(fmaxf(T.t0.x, T.t0.y), T.t0.z);
-            trb = fminf(fminf(T.t1.x, T.t1.y), T.t1.z);
-            stk[0] = packed;
-            return kTravLeaf;
-        }
+        if (steps >= budget) return kTravPending;
+        const int r = grid_step(cells, L, chx, T, stk, root, stride, leaf, tra, trb, touches);
+        if (r == kTravRevelles) return trav_next(chx, T, stk, root, stride, leaf, tra, trb, touches, budget - steps);
+        if (r != kTravPending) return r;
     }
 }
 

@@ -68,12 +68,10 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 #ifndef DRK_TRAV_BUDGET
 #define DRK_TRAV_BUDGET 4
 #endif
-// Cell steps of the leaf-grid walk per round: one (a step costs far less
-// than a Revelles node visit, and a longer loop only holds the wave's other
-// lanes; C1 with budgets 1/2/3/4: 29.5/31.2/32.8/33.6 ms).
-#ifndef DRK_GRID_BUDGET
-#define DRK_GRID_BUDGET 1
-#endif
+// The leaf-grid walk takes one cell step per round (grid_step): a step
+// costs far less than a Revelles node visit, and a longer loop only holds
+// the wave's other lanes (C1 with 1/2/3/4 steps per round:
+// 29.5/31.2/32.8/33.6 ms).
 #ifndef DRK_UNIT_RESERVE
 #define DRK_UNIT_RESERVE 32
 #endif
@@ -124,7 +122,6 @@ constexpr uint32_t kSpinLimit = 1u << 20;
 constexpr uint32_t kStallLimit = 1u << 22;
 constexpr uint16_t kSlotFree = 0xFFFFu;
 constexpr uint32_t kTravBudget = DRK_TRAV_BUDGET;
-constexpr uint32_t kGridBudget = DRK_GRID_BUDGET;
 constexpr uint32_t kEndBatch = DRK_END_BATCH;
 enum : uint32_t { RK_PRIMARY = 0, RK_CONT = 1, RK_SHADOW = 2, RK_SHADOW_PASS = 3 };
 
@@ -742,10 +739,14 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 T.lvl = -1;
                 found = true;
             } else {
-                const int r = T.lvl == kTravGrid
-                                  ? grid_next(P.cells, P.grid_levels, P.chx, T, stk, root, stride, leaf, tra, trbn,
-                                              cnt_nodes, kGridBudget)
-                                  : trav_next(P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
+                // One cell of the grid walk, or up to kTravBudget node
+                // visits of the Revelles walk (which also takes over from a
+                // grid step that met a tie); one inlined copy of each.
+                int r = kTravRevelles;
+                if (T.lvl == kTravGrid)
+                    r = grid_step(P.cells, P.grid_levels, P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes);
+                if (r == kTravRevelles)
+                    r = trav_next(P.chx, T, stk, root, stride, leaf, tra, trbn, cnt_nodes, kTravBudget);
                 if (r == kTravPending) continue_trav = true;
                 found = r == kTravLeaf;
             }

@@ -146,6 +146,7 @@ _SIGS = {
     "drk_probe_udiv": (C.c_int, [u32, u32, u32, C.c_uint64, C.POINTER(C.c_uint64)]),
     "drk_probe_hosek_state": (C.c_int, [f64, f64, f64, _D, _D]),
     "drk_probe_trilinear": (C.c_int, [_F, u32, _F, _F, u32, C.c_int, _F]),
+    "drk_probe_log_unit": (C.c_int, [u32, _F, _F]),
     "drk_probe_traverse": (C.c_int, [_P, _F, _F, u32, u32, _U, _F, _F, _U]),
     "drk_probe_camera": (C.c_int, [_F, _F, u32, u32, _U, _U, u32, _F, _F, _U]),
 }

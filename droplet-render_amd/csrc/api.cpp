@@ -477,6 +477,7 @@ int drk_kernel_create(const drk_scene *scene, const float viewi[16], const float
     P.pvol_sdf = scene->d_sdf;
     P.lvoxc = scene->lvoxc;
     P.lvoxc3 = scene->lvoxc * scene->lvoxc * scene->lvoxc;
+    P.lvoxm1 = (float)scene->lvoxc - 1.0f;
     P.leaf_rcp = scene->leaf_rcp;
     P.w = w;
     P.h = h;

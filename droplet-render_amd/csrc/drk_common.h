@@ -117,6 +117,7 @@ struct RenderParams {
     const float *depth;       // w*h compositing depth or nullptr
     const KernelConsts *kc;
     uint32_t lvoxc, lvoxc3;
+    float lvoxm1;             // lvoxc - 1 as a float (the lookup's cell scale)
     float leaf_rcp;           // RN(1 / leaf half-extent) when every leaf has the same extent, else 0
     uint32_t w, h;            // full image size (NDC)
     uint32_t x0, y0, rx, ry;  // tile origin and the rendered extent (even)

@@ -74,6 +74,18 @@ struct RayRng {
     __device__ __forceinline__ float uniform() { return u01(next()); }
 };
 
+// logf(u) for the tracker's uniforms u in [2^-24, 1): the device library's
+// logf for normal, finite arguments -- the hardware log2 (v_log_f32) scaled
+// by ln 2 split into a head and a tail with one fma each -- without its
+// denormal rescaling and non-finite selects, which these arguments never
+// take (a GPU test checks every possible uniform against logf bit for bit).
+__device__ __forceinline__ float log_unit(float u) {
+    const float r = __builtin_amdgcn_logf(u);
+    const float y = r * 0x1.62e42ep-1f;
+    const float t = __builtin_fmaf(r, 0x1.62e42ep-1f, -y);
+    return y + __builtin_fmaf(r, 0x1.efa39ep-25f, t);
+}
+
 // ---------------------------------------------------------------------------
 // Primary ray, K_ParallelRender (src/kernel.cpp:461-474): NDC with the
 // reference's sign flips, ro = viewi.r[3], rd = normalize((posh*proji).xyz0 * viewi).
@@ -443,10 +455,9 @@ __device__ __forceinline__ float markstein_div(float a, float e, float y) {
 }
 
 __device__ __forceinline__ float brick_lookup(const float *__restrict__ brick, F3 p, float4 ce, uint32_t lvoxc,
-                                              float rcp) {
+                                              float nvm, float rcp) {
 #pragma clang fp contract(off)
-    const float nv = (float)lvoxc;
-    const float nvm = nv - 1.0f;
+    // nvm = lvoxc - 1 (exact as a float), passed in rather than converted per lookup
     const float ax = (-0.5f * ((ce.x - ce.w) - p.x)) * nvm, ay = (-0.5f * ((ce.y - ce.w) - p.y)) * nvm,
                 az = (-0.5f * ((ce.z - ce.w) - p.z)) * nvm;
     float ni[3];

@@ -87,6 +87,14 @@ __global__ void k_mie_sample(const float *iv, const float *u1, const float *u2, 
     out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
 }
 
+__global__ void k_log_unit(uint32_t n, float *out_logf, float *out_fast) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float u = (float)((i << 1) | 1u) * 5.9604644775390625e-8f;
+    out_logf[i] = logf(u);
+    out_fast[i] = log_unit(u);
+}
+
 __global__ void k_hg_eval(float g, const float *ct, uint32_t n, float *out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = hg_eval(g, ct[i]);
@@ -128,7 +136,7 @@ __global__ void k_sky(SkyState S, const float *rd, uint32_t n, float *out) {
 __global__ void k_trilinear(const float *brick, uint32_t lvoxc, float4 ce, float rcp, const float *p, uint32_t n,
                             float *out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = brick_lookup(brick, mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), ce, lvoxc, rcp);
+    if (i < n) out[i] = brick_lookup(brick, mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), ce, lvoxc, (float)lvoxc - 1.0f, rcp);
 }
 
 __global__ __launch_bounds__(kProbeBlock) void k_traverse(const drk_octree_node *ob, const uint32_t *chx,
@@ -379,6 +387,21 @@ int drk_probe_sky(const drk_sun *sun, float turbidity, float albedo, const float
     k_sky<<<blocks(n), kProbeBlock>>>(S, drd, n, dout);
     PTRY(hipGetLastError(), "probe launch");
     PTRY(hipMemcpy(out_rgb, dout, 12 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
+    return DRK_OK;
+}
+
+int drk_probe_log_unit(uint32_t n, float *out_logf, float *out_fast) {
+    if (!out_logf || !out_fast) return set_error(DRK_EINVAL, "drk_probe_log_unit: NULL argument");
+    if (n > (1u << 23)) return set_error(DRK_EINVAL, "drk_probe_log_unit: n <= 2^23");
+    if (!n) return DRK_OK;
+    DevBufs B;
+    float *da, *db;
+    PTRY(B.alloc(da, n), "probe alloc");
+    PTRY(B.alloc(db, n), "probe alloc");
+    k_log_unit<<<blocks(n), kProbeBlock>>>(n, da, db);
+    PTRY(hipGetLastError(), "probe launch");
+    PTRY(hipMemcpy(out_logf, da, 4 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
+    PTRY(hipMemcpy(out_fast, db, 4 * (size_t)n, hipMemcpyDeviceToHost), "probe download");
     return DRK_OK;
 }
 

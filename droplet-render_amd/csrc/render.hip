@@ -774,7 +774,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     skip = false;
                     if (tra > td) {
                         F3 r0 = mk3(ro.x + rd.x * tra, ro.y + rd.y * tra, ro.z + rd.z * tra);
-                        skip = brick_lookup(P.pvol_sdf + (size_t)leaf_sdf * P.lvoxc3, r0, ce, P.lvoxc, P.leaf_rcp) > 0.0f;
+                        skip = brick_lookup(P.pvol_sdf + (size_t)leaf_sdf * P.lvoxc3, r0, ce, P.lvoxc, P.lvoxm1, P.leaf_rcp) > 0.0f;
                         if (kCount) cnt_lookups += 1;
                     }
                 } else {
@@ -804,7 +804,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         F3 ev_x0 = mk3(0, 0, 0), ev_x1 = mk3(0, 0, 0);
         float ev_w0 = 0.0f, ev_w1 = 0.0f;
         if (st == ST_LEAF) {
-            float step = logf(rng.uniform()) / kmaj;
+            float step = log_unit(rng.uniform()) / kmaj;
             sr = first_step ? -step : sr - step;
             first_step = false;
             float sc = s0 + sr;
@@ -823,13 +823,13 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 // (src/kernel.cpp:292-315).
                 float d = 1.0f, p;
                 if (leaf_sdf != DRK_NO_BRICK) {
-                    d = brick_lookup(P.pvol_sdf + (size_t)leaf_sdf * P.lvoxc3, rc, ce, P.lvoxc, P.leaf_rcp);
+                    d = brick_lookup(P.pvol_sdf + (size_t)leaf_sdf * P.lvoxc3, rc, ce, P.lvoxc, P.lvoxm1, P.leaf_rcp);
                     p = (d > 0.0f && leaf_brick != DRK_NO_BRICK)
-                            ? brick_lookup(P.pvol_fog + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc, P.leaf_rcp)
+                            ? brick_lookup(P.pvol_fog + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc, P.lvoxm1, P.leaf_rcp)
                             : -1.0f;
                     if (kCount) cnt_lookups += (d > 0.0f && leaf_brick != DRK_NO_BRICK) ? 2 : 1;
                 } else {
-                    p = brick_lookup(P.pvol_fog + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc, P.leaf_rcp);
+                    p = brick_lookup(P.pvol_fog + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc, P.lvoxm1, P.leaf_rcp);
                     if (kCount) cnt_lookups += 1;
                 }
                 float q = rng.uniform();
@@ -837,7 +837,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                 // same comparison; the ray's weight takes the ratio of the
                 // probabilities of that decision under the two densities
                 // (fog-only leaves: the weighted entry point refuses SDF).
-                const float pd = kWt ? brick_lookup(P.pvol_ref + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc, P.leaf_rcp) : p;
+                const float pd = kWt ? brick_lookup(P.pvol_ref + (size_t)leaf_brick * P.lvoxc3, rc, ce, P.lvoxc, P.lvoxm1, P.leaf_rcp) : p;
                 if (!(pd / smax < q) || !(d > 0.0f)) {
                     hit = true;
                     st = ST_END;

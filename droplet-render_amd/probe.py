@@ -34,6 +34,14 @@ def ray_stream(key0, sample, ray, key1, m):
     return out
 
 
+def log_unit(n=1 << 23):
+    """(logf(u), the kernels' log(u)) for u = (2i+1) 2^-24, i < n."""
+    a = np.zeros(n, np.float32)
+    b = np.zeros(n, np.float32)
+    L.check(L.lib().drk_probe_log_unit(int(n), L.fptr(a), L.fptr(b)))
+    return a, b
+
+
 def mie_eval(ct):
     ct = _f(ct)
     out = np.zeros((len(ct), 3), np.float32)

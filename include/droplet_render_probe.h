@@ -66,6 +66,11 @@ int drk_probe_traverse(const drk_scene *scene, const float *ro, const float *rd,
 int drk_probe_camera(const float viewi[16], const float proji[16], uint32_t w, uint32_t h, const uint32_t *px,
                      const uint32_t *py, uint32_t n, float *ro, float *rd, uint32_t *gm);
 
+/* The tracker's free-flight logarithm against the device library's logf:
+ * for u_i = (2i+1) * 2^-24, i < n (n <= 2^23: every uniform the RNG makes),
+ * out_logf[i] = logf(u_i) and out_fast[i] = the kernels' log of u_i. */
+int drk_probe_log_unit(uint32_t n, float *out_logf, float *out_fast);
+
 /* Host check of the kernels' division by an invariant divisor (the work-unit
  * to pixel mapping): counts the n in {n0 + i*step mod 2^32 : i < count} and
  * the 64 largest 32-bit values whose quotient by d differs from n / d.

@@ -148,6 +148,9 @@ struct drk_batch {
     uint32_t *d_counters = nullptr;  // kCounterWords words per job
     float *d_partial = nullptr;
     size_t partial_bytes = 0;
+    void *d_cam = nullptr;           // every job's camera rays (one float4 per pixel of its extent)
+    size_t cam_bytes = 0;
+    uint32_t cam_pix = 0;            // pixels of the largest extent
     uint32_t grid = 0;               // 0: occupancy-derived
     uint32_t grid_eff = 1;           // grid of the planned launch
     float unit_scale = 1.0f;         // sample chunks per pixel, relative to kBatchChunks
@@ -1249,6 +1252,25 @@ static int batch_plan(drk_batch *b) {
     for (uint32_t j = 0; j < nj; ++j)
         if (b->hW[j].nchunks > 1 && b->hW[j].npix)
             b->hW[j].partial = reinterpret_cast<float *>(reinterpret_cast<char *>(b->d_partial) + pbase[j]);
+    // Camera rays, once per pixel of every job (a pass before the render).
+    size_t coff = 0;
+    b->cam_pix = 0;
+    for (uint32_t j = 0; j < nj; ++j) {
+        coff += (size_t)b->hW[j].npix * 16;
+        b->cam_pix = std::max(b->cam_pix, b->hW[j].npix);
+    }
+    if (coff > b->cam_bytes) {
+        if (b->d_cam) (void)hipFree(b->d_cam);
+        b->d_cam = nullptr;
+        b->cam_bytes = 0;
+        HIP_TRY(dev_malloc(&b->d_cam, coff), "batch: camera rays");
+        b->cam_bytes = coff;
+    }
+    coff = 0;
+    for (uint32_t j = 0; j < nj; ++j) {
+        b->hW[j].cam = b->hW[j].npix ? reinterpret_cast<float4 *>(static_cast<char *>(b->d_cam) + coff) : nullptr;
+        coff += (size_t)b->hW[j].npix * 16;
+    }
     HIP_TRY(hipMemcpy(b->dP, b->hP.data(), sizeof(RenderParams) * nj, hipMemcpyHostToDevice), "batch: job table");
     HIP_TRY(hipMemcpy(b->dW, b->hW.data(), sizeof(WorkParams) * nj, hipMemcpyHostToDevice), "batch: job table");
     b->planned = true;
@@ -1275,7 +1297,7 @@ static int batch_enqueue(drk_batch *b, hipStream_t s) {
             }
         }
     }
-    HIP_TRY(launch_render_batch(b->dP, b->dW, nj, b->d_counters, nj * kCounterWords, b->grid_eff, mode, s),
+    HIP_TRY(launch_render_batch(b->dP, b->dW, nj, b->d_counters, nj * kCounterWords, b->grid_eff, mode, b->cam_pix, s),
             "batch: launch");
     for (uint32_t j = 0; j < nj; ++j)
         HIP_TRY(launch_reduce(b->hW[j], b->hP[j].rx, b->hP[j].tilex, b->sumsq ? 4u : 2u, s), "batch: reduce");
@@ -1430,6 +1452,7 @@ void drk_batch_destroy(drk_batch *b) {
     if (b->dW) (void)hipFree(b->dW);
     if (b->d_counters) (void)hipFree(b->d_counters);
     if (b->d_partial) (void)hipFree(b->d_partial);
+    if (b->d_cam) (void)hipFree(b->d_cam);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     delete b;
 }

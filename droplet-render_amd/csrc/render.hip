@@ -1232,17 +1232,21 @@ __global__ void k_launch_setup(RenderParams P, WorkParams W) {
     }
 }
 
-// The camera ray of every pixel of the rendered extent (W.cam).
-__global__ void k_camera_rays(const LaunchParams *__restrict__ L) {
-    const RenderParams &P = L->P;
-    const WorkParams &W = L->W;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W.npix) return;
+// The camera ray of every pixel of the rendered extent (W.cam): one launch,
+// or one row of blocks per job of a batch.
+__device__ __forceinline__ void camera_ray_of(const RenderParams &P, const WorkParams &W, uint32_t i) {
+    if (i >= W.npix || !W.cam) return;
     const uint32_t ly = udiv(i, W.rx_div), lx = i - ly * P.rx;
     F3 ro, rd;
     bool gm;
     camera_ray(*P.kc, P.w, P.h, P.x0 + lx, P.y0 + ly, ro, rd, gm);
     W.cam[i] = make_float4(rd.x, rd.y, rd.z, gm ? 1.0f : 0.0f);
+}
+__global__ void k_camera_rays(const LaunchParams *__restrict__ L) {
+    camera_ray_of(L->P, L->W, blockIdx.x * blockDim.x + threadIdx.x);
+}
+__global__ void k_camera_rays_batch(const RenderParams *__restrict__ Ps, const WorkParams *__restrict__ Ws) {
+    camera_ray_of(Ps[blockIdx.y], Ws[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -1289,9 +1293,11 @@ int render_batch_blocks_per_cu(int mode) {
 }
 
 hipError_t launch_render_batch(const RenderParams *dP, const WorkParams *dW, uint32_t njobs, uint32_t *counters,
-                               uint32_t ncounters, uint32_t grid, int mode, hipStream_t s) {
+                               uint32_t ncounters, uint32_t grid, int mode, uint32_t cam_pix, hipStream_t s) {
     if (njobs == 0) return hipSuccess;
     k_fill_u32<<<(ncounters + 255) / 256, 256, 0, s>>>(counters, ncounters, 0u);
+    for (uint32_t j0 = 0; cam_pix && j0 < njobs; j0 += 65535u)
+        k_camera_rays_batch<<<dim3((cam_pix + 255) / 256, min(65535u, njobs - j0)), 256, 0, s>>>(dP + j0, dW + j0);
     if (mode == kModeForwardSq) k_render_batch<kModeForwardSq><<<grid, kBlock, 0, s>>>(dP, dW, njobs);
     else k_render_batch<kModeForward><<<grid, kBlock, 0, s>>>(dP, dW, njobs);
     return hipGetLastError();

@@ -936,7 +936,10 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     path_done = shadow_end();
             } else if (!hit) {
                 if (depth_r > 0 && ((kAdj && pass == 0) || kRec)) last_c = (int32_t)depth_r;
-                if (depth_r > 0) {
+                // The adjoint's gradient pass takes the path's radiance from
+                // pass 0 or the record, so it does not evaluate the escape's
+                // sun and sky again.
+                if (depth_r > 0 && !(kAdj && pass == 1)) {
                     // The escape service takes the radiance when this path
                     // ends a unit whose sums are zero so far and the lane's
                     // previous escape job has been taken.

@@ -207,7 +207,8 @@ int drk_kernel_render_device(drk_kernel *k, uint32_t x0, uint32_t y0,
  * call (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). Same arguments, work and results as
  * drk_kernel_render_device with the same chunking, but the launch's scratch
- * (its work-unit counters and per-chunk partial sums) lives in a fixed
+ * (its work-unit counters, launch parameters and per-chunk partial sums; the
+ * camera rays are computed per path instead of once per pixel) lives in a fixed
  * 128 MiB device array that the library holds on every GPU it runs on until
  * the process ends, so the call allocates nothing. A call whose partial sums
  * exceed that array (pixels x sample chunks x 32 bytes, 64 with sums of

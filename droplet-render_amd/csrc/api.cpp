@@ -95,7 +95,7 @@ struct drk_scene {
     size_t ev_budget = SIZE_MAX;    // SIZE_MAX: a quarter of device memory
     uint32_t ev_cap = 0;
     void *d_ev = nullptr;
-    uint32_t *d_evbins = nullptr;   // count (2 words) + hist + bin_start + bin_fill + blk_start
+    uint32_t *d_evbins = nullptr;   // count (2 words) + hist + bin_start + bin_fill + blk_start + bin_max
 };
 
 struct drk_kernel {
@@ -740,7 +740,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             // of float additions), so an allocation failure is not an error.
             hipError_t e1 = dev_malloc((void **)&sc->d_ev, (size_t)cap * kEventSlotBytes);
             hipError_t e2 = e1 == hipSuccess
-                                ? dev_malloc((void **)&sc->d_evbins, (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t))
+                                ? dev_malloc((void **)&sc->d_evbins, kEvTableWords * sizeof(uint32_t))
                                 : e1;
             if (e1 == hipSuccess && e2 == hipSuccess) {
                 sc->ev_cap = (uint32_t)cap;
@@ -1075,7 +1075,7 @@ int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
     *bytes = (size_t)s->node_count * (sizeof(drk_octree_node) + 8 * sizeof(uint32_t)) +
              ((size_t)s->leafx[0] + s->leafx[1]) * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float) +
              (s->d_bce ? (size_t)s->leafx[1] * 4 * sizeof(float) : 0);
-    if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + (2 + 4 * (size_t)kEvMaxBins + 2) * sizeof(uint32_t);
+    if (s->d_ev) *bytes += (size_t)s->ev_cap * kEventSlotBytes + kEvTableWords * sizeof(uint32_t);
     if (s->d_ref) *bytes += (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc * sizeof(float);
     if (s->d_cells) *bytes += ((size_t)1 << (3 * s->grid_levels)) * sizeof(uint32_t);
     return DRK_OK;

@@ -1372,8 +1372,11 @@ __device__ __forceinline__ uint32_t ev_n(const EventReduce &E) {
 }
 
 // Bin tables after the 2-word count.
+// bin_max: the largest |weight| of each bin's events, as float bits (which
+// order like the values for non-negative floats; above 0x7f7fffff: an
+// infinite or NaN weight), gathered by the scatter for the accumulation.
 struct BinTables {
-    uint32_t *hist, *bin_start, *bin_fill, *blk_start;
+    uint32_t *hist, *bin_start, *bin_fill, *blk_start, *bin_max;
 };
 __device__ __forceinline__ BinTables bin_tables(const EventReduce &E) {
     BinTables t;
@@ -1381,6 +1384,7 @@ __device__ __forceinline__ BinTables bin_tables(const EventReduce &E) {
     t.bin_start = t.hist + E.nbins;
     t.bin_fill = t.bin_start + E.nbins + 1;
     t.blk_start = t.bin_fill + E.nbins;
+    t.bin_max = t.blk_start + E.nbins + 1;
     return t;
 }
 
@@ -1472,17 +1476,20 @@ __device__ __forceinline__ void block_exscan(uint32_t *a, uint32_t n, uint32_t *
 // slots of a bin's run. Stored straight from their lanes (one cache line per
 // lane), the 16-byte events kept the texture addresser busy 80% of the
 // kernel's time (C1: 9.4 ms; through the tiles 7.2 ms; tiles of 3072 or 6144
-// events measured no faster). LDS: two words per bin (<= 8192 bins) and
-// 20 B per tile slot, at most 104 KB.
+// events measured no faster). The scatter also gathers each bin's largest
+// |weight| (an LDS maximum per event, one global maximum per bin and block)
+// so that the accumulation needs no pass of its own for the fixed-point
+// scale. LDS: three words per bin (<= 8192 bins) and 20 B per tile slot, at
+// most 136 KB.
 constexpr uint32_t kEvTile = 2048;
 __host__ __device__ constexpr size_t ev_scatter_lds(uint32_t nbins) {
-    return ((size_t)2 * nbins + 16 + 3) / 4 * 16 + (size_t)kEvTile * (16 + 4);
+    return ((size_t)3 * nbins + 16 + 3) / 4 * 16 + (size_t)kEvTile * (16 + 4);
 }
 __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
     extern __shared__ uint32_t lds[];
     const uint32_t nb = E.nbins;
-    uint32_t *cur = lds, *th = lds + nb, *wsum = lds + 2 * nb;
-    float4 *tw = reinterpret_cast<float4 *>(lds + (2 * nb + 16 + 3) / 4 * 4);
+    uint32_t *cur = lds, *th = lds + nb, *tmx = lds + 2 * nb, *wsum = lds + 3 * nb;
+    float4 *tw = reinterpret_cast<float4 *>(lds + (3 * nb + 16 + 3) / 4 * 4);
     uint32_t *tb = reinterpret_cast<uint32_t *>(tw + kEvTile);
     const uint32_t n = ev_n(E);
     const uint64_t start = (uint64_t)blockIdx.x * kEvSortChunk;
@@ -1490,7 +1497,7 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
     const uint32_t end = (uint32_t)min<uint64_t>(start + kEvSortChunk, n);
     const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
     const BinTables t = bin_tables(E);
-    for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock) th[b] = 0;
+    for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock) th[b] = tmx[b] = 0;
     __syncthreads();
     for (uint32_t i = (uint32_t)start + threadIdx.x; i < end; i += kEvSortBlock) atomicAdd(&th[ev.b[i] >> E.shift], 1u);
     __syncthreads();
@@ -1511,6 +1518,7 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
                 br[u] = ev.b[t0 + j];
                 w[u] = ev.wx[t0 + j];
                 rk[u] = atomicAdd(&th[br[u] >> E.shift], 1u);
+                atomicMax(&tmx[br[u] >> E.shift], __float_as_uint(fabsf(w[u].x)));
             }
         }
         __syncthreads();
@@ -1541,6 +1549,9 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
         for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock)
             cur[b] += (b + 1 < nb ? th[b + 1] : tn) - th[b];
     }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += kEvSortBlock)
+        if (tmx[b]) atomicMax(&t.bin_max[b], tmx[b]);
 }
 
 // One block per (bin, chunk of at most kEvAccChunk events): sums the chunk's
@@ -1548,15 +1559,14 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
 // non-zero sums to the gradient. The LDS sums are 64-bit fixed point: LDS
 // integer atomics run about ten times faster than ds_add_f32, and integer
 // sums do not depend on the order the lanes add in. The scale is a power of
-// two from the chunk's largest |weight| M: every splat (|weight * w| <= M,
-// w <= 1) is rounded to a multiple of 2^-S with M * 2^S < 2^45, so the at
-// most 2^17 splats of a chunk stay below 2^62, and a splat loses at most
-// 2^-46 M to rounding (float sums lose 2^-24 of the running sum per add).
-// A chunk holding a non-finite weight takes a float path instead, so that
-// the NaN or infinity reaches the gradient.
+// two from the largest |weight| M of the chunk's bin (gathered by the
+// scatter): every splat (|weight * w| <= M, w <= 1) is rounded to a multiple
+// of 2^-S with M * 2^S < 2^45, so the at most 2^17 splats of a chunk stay
+// below 2^62, and a splat loses at most 2^-46 M to rounding (float sums lose
+// 2^-24 of the running sum per add). A bin holding a non-finite weight takes
+// a float path instead, so that the NaN or infinity reaches the gradient.
 __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     extern __shared__ unsigned long long lq[];
-    __shared__ uint32_t s_max, s_bad;
     const EventPlanes ev = event_planes(E.ev_base, E.ev_cap);
     const BinTables t = bin_tables(E);
     const uint32_t total = t.blk_start[E.nbins];
@@ -1573,23 +1583,15 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     const uint32_t e1 = min(e0 + kEvAccChunk, t.bin_start[b + 1]);
     const uint32_t brick0 = b << E.shift;
     const uint32_t span = min(1u << E.shift, E.nbricks - brick0) * E.lvoxc3;
-    if (threadIdx.x == 0) s_max = s_bad = 0u;
     for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lq[j] = 0ull;
-    // Largest |weight| of the chunk (as bits: ordered like the values for
-    // non-negative floats) and whether any weight is not finite.
-    uint32_t mbits = 0u, bad = 0u;
-    for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
-        const float a = fabsf(ev.swx[i].x);
-        bad |= !(a <= 3.402823466e38f);
-        mbits = max(mbits, __float_as_uint(a));
-    }
+    // Largest |weight| of the bin (from the scatter) and whether any weight
+    // of the bin is not finite.
+    const uint32_t mb = t.bin_max[b];
+    if (mb == 0u) return;   // all weights zero: nothing to add
+    const bool bad = mb > 0x7f7fffffu;
     __syncthreads();
-    if (bad) atomicOr(&s_bad, 1u);
-    atomicMax(&s_max, mbits);
-    __syncthreads();
-    if (s_max == 0u) return;   // all weights zero: nothing to add
     float *out = E.grad + (size_t)brick0 * E.lvoxc3;
-    if (s_bad) {
+    if (bad) {
         float *lg = reinterpret_cast<float *>(lq);
         for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lg[j] = 0.0f;
         __syncthreads();
@@ -1614,7 +1616,7 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
         }
         return;
     }
-    const int sh = 44 - ilogbf(__uint_as_float(s_max));   // M * 2^sh < 2^45
+    const int sh = 44 - ilogbf(__uint_as_float(mb));   // M * 2^sh < 2^45
     for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
         const float4 wx = ev.swx[i];
         const uint32_t brick = ev.sb[i];
@@ -1642,6 +1644,8 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     }
 }
 
+static uint32_t *bin_max_of(const EventReduce &E) { return E.bins + 2 + 4 * (size_t)E.nbins + 2; }
+
 bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nbins) {
     shift = 0;
     while (((uint64_t)nbricks + (1ull << shift) - 1) >> shift > kEvMaxBins) ++shift;
@@ -1658,6 +1662,7 @@ hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
     const uint32_t sort_blocks = (uint32_t)(((uint64_t)E.ev_cap + kEvSortChunk - 1) / kEvSortChunk);
     const uint32_t acc_blocks = (uint32_t)(((uint64_t)E.ev_cap + kEvAccChunk - 1) / kEvAccChunk) + E.nbins;
     k_fill_u32<<<(E.nbins + 255) / 256, 256, 0, s>>>(E.bins + 2, E.nbins, 0u);
+    k_fill_u32<<<(E.nbins + 255) / 256, 256, 0, s>>>(bin_max_of(E), E.nbins, 0u);
     k_ev_hist<<<sort_blocks, kEvSortBlock, E.nbins * sizeof(uint32_t), s>>>(E);
     k_ev_scan<<<1, 1024, 0, s>>>(E);
     k_ev_scatter<<<sort_blocks, kEvSortBlock, ev_scatter_lds(E.nbins), s>>>(E);

@@ -198,7 +198,7 @@ hipError_t launch_grad_sum(const float *part, uint32_t copies, size_t n, float *
 struct EventReduce {
     char *ev_base;         // event buffer (see EventPlanes)
     uint32_t *bins;        // bin tables: count (2 words), hist[nbins], bin_start[nbins+1],
-                           // bin_fill[nbins], blk_start[nbins+1]
+                           // bin_fill[nbins], blk_start[nbins+1], bin_max[nbins]
     uint32_t ev_cap;
     uint32_t nbins, shift; // bin = brick >> shift
     uint32_t nbricks, lvoxc, lvoxc3;
@@ -226,6 +226,7 @@ __host__ __device__ inline EventPlanes event_planes(char *base, uint32_t cap) {
     return e;
 }
 constexpr uint32_t kEvMaxBins = 8192;          // LDS histogram of the sort
+constexpr size_t kEvTableWords = 2 + 5 * (size_t)kEvMaxBins + 2;   // the bin tables (EventReduce::bins)
 constexpr size_t kEventSlotBytes = 40;         // event (20 B) + its sorted copy (20 B)
 constexpr uint32_t kEvChunk = 2048;            // event slots a wave reserves at a time
 constexpr uint32_t kEvAccumLds = 65536;        // bytes of LDS gradient per accumulation block

@@ -761,6 +761,11 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             E.ev_cap = c;
             E.nbins = nbins;
             E.shift = shift;
+            // Accumulation blocks of 2^14 events, or 2^12 for calls of fewer
+            // than 2^24 samples, whose few blocks would otherwise leave CUs
+            // idle (C1: 2^14 3.38 ms, 2^12 3.72 ms; a C3 view: fit step 42.1 ms
+            // at 2^14, 40.8 ms at 2^12).
+            E.acc_log2 = (uint64_t)npix * samples < (1ull << 24) ? 12u : 14u;
             E.nbricks = k->scene->leafx[1];
             E.lvoxc = k->P.lvoxc;
             E.lvoxc3 = k->P.lvoxc3;

@@ -1363,7 +1363,6 @@ hipError_t launch_grad_sum(const float *part, uint32_t copies, size_t n, float *
 // ---------------------------------------------------------------------------
 constexpr uint32_t kEvSortChunk = 65536;   // events per sort block
 constexpr uint32_t kEvSortBlock = 1024;
-constexpr uint32_t kEvAccChunk = 16384;    // events per accumulation block
 constexpr uint32_t kEvAccBlock = 256;
 
 __device__ __forceinline__ uint32_t ev_n(const EventReduce &E) {
@@ -1414,7 +1413,7 @@ __global__ __launch_bounds__(1024) void k_ev_scan(EventReduce E) {
     uint32_t se = 0, sb = 0;
     for (uint32_t b = b0; b < b1; ++b) {
         se += T.hist[b];
-        sb += (T.hist[b] + kEvAccChunk - 1) / kEvAccChunk;
+        sb += (T.hist[b] + (1u << E.acc_log2) - 1) >> E.acc_log2;
     }
     part_e[threadIdx.x] = se;
     part_b[threadIdx.x] = sb;
@@ -1439,7 +1438,7 @@ __global__ __launch_bounds__(1024) void k_ev_scan(EventReduce E) {
         T.blk_start[b] = sb;
         T.bin_fill[b] = 0;
         se += T.hist[b];
-        sb += (T.hist[b] + kEvAccChunk - 1) / kEvAccChunk;
+        sb += (T.hist[b] + (1u << E.acc_log2) - 1) >> E.acc_log2;
     }
 }
 
@@ -1554,16 +1553,17 @@ __global__ __launch_bounds__(kEvSortBlock) void k_ev_scatter(EventReduce E) {
         if (tmx[b]) atomicMax(&t.bin_max[b], tmx[b]);
 }
 
-// One block per (bin, chunk of at most kEvAccChunk events): sums the chunk's
+// One block per (bin, chunk of at most 2^acc_log2 events): sums the chunk's
 // trilinear splats into an LDS copy of the bin's bricks, then adds the
 // non-zero sums to the gradient. The LDS sums are 64-bit fixed point: LDS
 // integer atomics run about ten times faster than ds_add_f32, and integer
 // sums do not depend on the order the lanes add in. The scale is a power of
 // two from the largest |weight| M of the chunk's bin (gathered by the
 // scatter): every splat (|weight * w| <= M, w <= 1) is rounded to a multiple
-// of 2^-S with M * 2^S < 2^45, so the at most 2^17 splats of a chunk stay
-// below 2^62, and a splat loses at most 2^-46 M to rounding (float sums lose
-// 2^-24 of the running sum per add). A bin holding a non-finite weight takes
+// of 2^-S with M * 2^S < 2^(59 - acc_log2), so the at most 2^(acc_log2 + 3)
+// splats of a chunk stay below 2^62, and a splat loses at most
+// 2^(-60 + acc_log2) M to rounding (2^-46 M for chunks of 2^14 events; float
+// sums lose 2^-24 of the running sum per add). A bin holding a non-finite weight takes
 // a float path instead, so that the NaN or infinity reaches the gradient.
 __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
     extern __shared__ unsigned long long lq[];
@@ -1579,8 +1579,8 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
         else hi = mid;
     }
     const uint32_t b = lo;
-    const uint32_t e0 = t.bin_start[b] + (blockIdx.x - t.blk_start[b]) * kEvAccChunk;
-    const uint32_t e1 = min(e0 + kEvAccChunk, t.bin_start[b + 1]);
+    const uint32_t e0 = t.bin_start[b] + ((blockIdx.x - t.blk_start[b]) << E.acc_log2);
+    const uint32_t e1 = min(e0 + (1u << E.acc_log2), t.bin_start[b + 1]);
     const uint32_t brick0 = b << E.shift;
     const uint32_t span = min(1u << E.shift, E.nbricks - brick0) * E.lvoxc3;
     for (uint32_t j = threadIdx.x; j < span; j += kEvAccBlock) lq[j] = 0ull;
@@ -1616,7 +1616,7 @@ __global__ __launch_bounds__(kEvAccBlock) void k_ev_accum(EventReduce E) {
         }
         return;
     }
-    const int sh = 44 - ilogbf(__uint_as_float(mb));   // M * 2^sh < 2^45
+    const int sh = 58 - (int)E.acc_log2 - ilogbf(__uint_as_float(mb));   // M * 2^sh < 2^(59 - acc_log2)
     for (uint32_t i = e0 + threadIdx.x; i < e1; i += kEvAccBlock) {
         const float4 wx = ev.swx[i];
         const uint32_t brick = ev.sb[i];
@@ -1660,7 +1660,7 @@ bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nb
 hipError_t launch_event_reduce(const EventReduce &E, hipStream_t s) {
     if (E.ev_cap == 0 || E.nbins == 0) return hipSuccess;
     const uint32_t sort_blocks = (uint32_t)(((uint64_t)E.ev_cap + kEvSortChunk - 1) / kEvSortChunk);
-    const uint32_t acc_blocks = (uint32_t)(((uint64_t)E.ev_cap + kEvAccChunk - 1) / kEvAccChunk) + E.nbins;
+    const uint32_t acc_blocks = (uint32_t)(((uint64_t)E.ev_cap + (1u << E.acc_log2) - 1) >> E.acc_log2) + E.nbins;
     k_fill_u32<<<(E.nbins + 255) / 256, 256, 0, s>>>(E.bins + 2, E.nbins, 0u);
     k_fill_u32<<<(E.nbins + 255) / 256, 256, 0, s>>>(bin_max_of(E), E.nbins, 0u);
     k_ev_hist<<<sort_blocks, kEvSortBlock, E.nbins * sizeof(uint32_t), s>>>(E);

@@ -201,6 +201,7 @@ struct EventReduce {
                            // bin_fill[nbins], blk_start[nbins+1], bin_max[nbins]
     uint32_t ev_cap;
     uint32_t nbins, shift; // bin = brick >> shift
+    uint32_t acc_log2;     // events per accumulation block: 2^acc_log2 (12..14)
     uint32_t nbricks, lvoxc, lvoxc3;
     const float4 *bce;     // per FOG brick: centre and half-size of its leaf
     float *grad;           // accumulated into

@@ -120,6 +120,8 @@ _SIGS = {
     "drk_kernel_adjoint": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _F, _F, _F]),
     "drk_kernel_adjoint_device": (C.c_int, [_P, u32, u32, u32, u32, u32, u64, u32, _P, _P, _P, _P]),
     "drk_kernel_set_event_budget": (C.c_int, [_P, u64]),
+    "drk_scene_events_begin": (C.c_int, [_P, u64, _P]),
+    "drk_scene_events_end": (C.c_int, [_P, _P, _P]),
     "drk_kernel_footprint": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
     "drk_scene_footprint": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
     "drk_kernel_record_bytes": (C.c_int, [_P, u32, u32, u32, u32, u32, C.POINTER(C.c_size_t)]),

@@ -61,6 +61,76 @@ class _Render(torch.autograd.Function):
         return grad, None, None, None, None, None, None, None, None, None
 
 
+class _RenderViews(torch.autograd.Function):
+    """Several full-frame renders of one scene (views) as one node: the
+    forward renders every view, the backward runs every view's adjoint into
+    one gradient inside the scene's gradient window, so their events are
+    reduced once."""
+
+    @staticmethod
+    def forward(ctx, density, kernels, samples, seeds, record):
+        scene = kernels[0].scene
+        assert all(k.scene is scene for k in kernels)
+        assert density.is_cuda and density.dtype == torch.float32 and density.is_contiguous()
+        assert density.numel() == scene.leafx[1] * scene.lvoxc3
+        torch.cuda.current_stream(density.device).synchronize()
+        scene.update_fog(density)
+        stream = torch.cuda.current_stream(density.device)
+        outs, recs = [], []
+        for k, seed in zip(kernels, seeds):
+            n = k.tilex * k.tiley * 4
+            out_l = torch.empty(n, dtype=torch.float32, device=density.device)
+            out_e = torch.empty(n, dtype=torch.float32, device=density.device)
+            if record:
+                nb = k.record_bytes(0, 0, k.tilex, k.tiley, samples)
+                rec = torch.empty((nb + 15) // 16 * 4, dtype=torch.float32, device=density.device)
+                k.render_record_device(0, 0, k.tilex, k.tiley, samples, seed, 0, out_l, out_e, rec, stream=stream)
+                recs.append(rec)
+            else:
+                k.render_device(0, 0, k.tilex, k.tiley, samples, seed, 0, out_l, out_e, stream=stream)
+            outs += [out_l.view(k.tiley, k.tilex, 4), out_e.view(k.tiley, k.tilex, 4)]
+        ctx.kernels, ctx.samples, ctx.seeds, ctx.recs = kernels, samples, seeds, recs if record else None
+        ctx.save_for_backward(density)
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        (density,) = ctx.saved_tensors
+        kernels, samples = ctx.kernels, ctx.samples
+        scene = kernels[0].scene
+        torch.cuda.current_stream(density.device).synchronize()
+        scene.update_fog(density)
+        stream = torch.cuda.current_stream(density.device)
+        grad = torch.zeros_like(density)
+        total = sum(k.tilex * k.tiley * samples for k in kernels)
+        scene.events_begin(total, stream=stream)
+        try:
+            for i, (k, seed) in enumerate(zip(kernels, ctx.seeds)):
+                g_l, g_e = grads[2 * i], grads[2 * i + 1]
+                if g_l is None and g_e is None:
+                    continue
+                shape = (k.tiley, k.tilex, 4)
+                g_l = torch.zeros(shape, device=density.device) if g_l is None else g_l.contiguous().float()
+                g_e = torch.zeros(shape, device=density.device) if g_e is None else g_e.contiguous().float()
+                if ctx.recs is not None:
+                    k.adjoint_record_device(0, 0, k.tilex, k.tiley, samples, seed, 0, g_l, g_e, grad, ctx.recs[i],
+                                            stream=stream)
+                else:
+                    k.adjoint_device(0, 0, k.tilex, k.tiley, samples, seed, 0, g_l, g_e, grad, stream=stream)
+        finally:
+            scene.events_end(grad, stream=stream)
+        ctx.recs = None
+        return grad, None, None, None, None
+
+
+def render_views(kernels, density, samples, seeds, record=True):
+    """Differentiable full-frame renders of several kernels of one scene:
+    [(lights sums, environment sums)] per kernel, each (tiley, tilex, 4).
+    The backward adds every view's gradient with one event reduction."""
+    outs = _RenderViews.apply(density, tuple(kernels), int(samples), tuple(int(s) for s in seeds), bool(record))
+    return [(outs[2 * i], outs[2 * i + 1]) for i in range(len(kernels))]
+
+
 def render(kernel, density, samples=None, seed=0, sample_base=0, x0=0, y0=0, tilex=None, tiley=None, record=True):
     """Differentiable render: (lights sums, environment sums), each (tiley, tilex, 4)."""
     tilex = tilex or kernel.tilex
@@ -70,4 +140,4 @@ def render(kernel, density, samples=None, seed=0, sample_base=0, x0=0, y0=0, til
                          int(sample_base), bool(record))
 
 
-__all__ = ["render", "L"]
+__all__ = ["render", "render_views", "L"]

@@ -96,6 +96,12 @@ struct drk_scene {
     uint32_t ev_cap = 0;
     void *d_ev = nullptr;
     uint32_t *d_evbins = nullptr;   // count (2 words) + hist + bin_start + bin_fill + blk_start + bin_max
+    // Gradient window (drk_scene_events_begin/end): adjoint calls append
+    // their events to the buffer's first ev_window_cap slots and the
+    // reduction runs once when the window ends.
+    bool ev_window = false;
+    uint32_t ev_window_cap = 0;
+    uint64_t ev_window_samples = 0;
 };
 
 struct drk_kernel {
@@ -619,6 +625,72 @@ static void rendered_extent(const drk_kernel *k, uint32_t x0, uint32_t y0, uint3
     ry = y0 < k->P.h ? std::min(tiley & ~1u, k->P.h - y0) : 0u;
 }
 
+// Event buffer of the scene's adjoint calls: 40 bytes per slot (event +
+// sorted copy), sized to 16 events per sample of `samples` within the
+// budget (a multiple of 64 slots, which keeps every plane 16-byte aligned);
+// kept for later calls and reallocated when a call wants more. Returns the
+// slots to use (0: no events; every contribution then goes to the gradient
+// with atomics, the same gradient up to the order of float additions, so an
+// allocation failure is not an error).
+static uint32_t ensure_event_buffer(drk_scene *sc, uint64_t samples) {
+    uint32_t shift = 0, nbins = 0;
+    const bool binnable = event_bins(sc->leafx[1], sc->lvoxc * sc->lvoxc * sc->lvoxc, shift, nbins) && sc->bce_ok;
+    if (!binnable) return 0;
+    size_t budget = sc->ev_budget;
+    if (budget == SIZE_MAX) {
+        // A quarter of the memory that is free now (other allocators,
+        // torch's caching one among them, may hold much of the device),
+        // plus what the current buffer already holds.
+        size_t fr = 0, tot = 0;
+        const size_t held = sc->d_ev ? (size_t)sc->ev_cap * kEventSlotBytes : 0;
+        budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (std::min(fr + held, tot)) / 4 : 0;
+    }
+    const uint64_t want = std::min<uint64_t>(samples * 16, 0xffffffffull);
+    const uint64_t cap = std::min<uint64_t>(want, budget / kEventSlotBytes) & ~63ull;
+    if (cap > sc->ev_cap) {
+        if (sc->d_ev) (void)hipFree(sc->d_ev);
+        if (sc->d_evbins) (void)hipFree(sc->d_evbins);
+        sc->d_ev = nullptr;
+        sc->d_evbins = nullptr;
+        sc->ev_cap = 0;
+        hipError_t e1 = dev_malloc((void **)&sc->d_ev, (size_t)cap * kEventSlotBytes);
+        hipError_t e2 = e1 == hipSuccess ? dev_malloc((void **)&sc->d_evbins, kEvTableWords * sizeof(uint32_t)) : e1;
+        if (e1 == hipSuccess && e2 == hipSuccess) {
+            sc->ev_cap = (uint32_t)cap;
+        } else {
+            (void)hipGetLastError();
+            if (sc->d_ev) (void)hipFree(sc->d_ev);
+            sc->d_ev = nullptr;
+            sc->d_evbins = nullptr;
+        }
+    }
+    return cap > 0 && sc->ev_cap > 0 ? sc->ev_cap : 0u;
+}
+
+// Parameters of the event reduction over `cap` slots of the scene's buffer
+// into grad, for calls that rendered `samples` pixel samples in total.
+static void event_reduce_params(const drk_scene *sc, uint32_t cap, uint64_t samples, float *grad, EventReduce &E) {
+    uint32_t shift = 0, nbins = 0;
+    const uint32_t lvoxc3 = sc->lvoxc * sc->lvoxc * sc->lvoxc;
+    (void)event_bins(sc->leafx[1], lvoxc3, shift, nbins);
+    std::memset(&E, 0, sizeof(E));
+    E.ev_base = static_cast<char *>(sc->d_ev);
+    E.bins = sc->d_evbins;
+    E.ev_cap = cap;
+    E.nbins = nbins;
+    E.shift = shift;
+    // Accumulation blocks of 2^14 events, or 2^12 for calls of fewer
+    // than 2^24 samples, whose few blocks would otherwise leave CUs
+    // idle (C1: 2^14 3.38 ms, 2^12 3.72 ms; a C3 view: fit step 42.1 ms
+    // at 2^14, 40.8 ms at 2^12).
+    E.acc_log2 = samples < (1ull << 24) ? 12u : 14u;
+    E.nbricks = sc->leafx[1];
+    E.lvoxc = sc->lvoxc;
+    E.lvoxc3 = lvoxc3;
+    E.bce = reinterpret_cast<const float4 *>(sc->d_bce);
+    E.grad = grad;
+}
+
 static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
                        hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr,
@@ -710,71 +782,22 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     const size_t gn = (size_t)k->scene->leafx[1] * k->P.lvoxc3;
     EventReduce E;
     std::memset(&E, 0, sizeof(E));
+    drk_scene *sc = k->scene;
     if (adj) {
-        // Event buffer: 40 bytes per slot (event + sorted copy), sized to 16
-        // events per sample of this call within the budget (a multiple of 64
-        // slots, which keeps every plane 16-byte aligned); kept for later
-        // calls and reallocated when a call wants more.
-        uint32_t shift = 0, nbins = 0;
-        drk_scene *sc = k->scene;
-        const bool binnable = event_bins(sc->leafx[1], k->P.lvoxc3, shift, nbins) && sc->bce_ok;
-        size_t budget = sc->ev_budget;
-        if (budget == SIZE_MAX) {
-            // A quarter of the memory that is free now (other allocators,
-            // torch's caching one among them, may hold much of the device),
-            // plus what the current buffer already holds.
-            size_t fr = 0, tot = 0;
-            const size_t held = sc->d_ev ? (size_t)sc->ev_cap * kEventSlotBytes : 0;
-            budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (std::min(fr + held, tot)) / 4 : 0;
-        }
-        const uint64_t want = std::min<uint64_t>((uint64_t)npix * samples * 16, 0xffffffffull);
-        const uint64_t cap = binnable ? (std::min<uint64_t>(want, budget / kEventSlotBytes) & ~63ull) : 0;
-        if (cap > sc->ev_cap) {
-            if (sc->d_ev) (void)hipFree(sc->d_ev);
-            if (sc->d_evbins) (void)hipFree(sc->d_evbins);
-            sc->d_ev = nullptr;
-            sc->d_evbins = nullptr;
-            sc->ev_cap = 0;
-            // Without room for events every contribution goes to the
-            // gradient with atomics instead (same gradient up to the order
-            // of float additions), so an allocation failure is not an error.
-            hipError_t e1 = dev_malloc((void **)&sc->d_ev, (size_t)cap * kEventSlotBytes);
-            hipError_t e2 = e1 == hipSuccess
-                                ? dev_malloc((void **)&sc->d_evbins, kEvTableWords * sizeof(uint32_t))
-                                : e1;
-            if (e1 == hipSuccess && e2 == hipSuccess) {
-                sc->ev_cap = (uint32_t)cap;
-            } else {
-                (void)hipGetLastError();
-                if (sc->d_ev) (void)hipFree(sc->d_ev);
-                sc->d_ev = nullptr;
-                sc->d_evbins = nullptr;
-            }
-        }
-        if (cap > 0 && sc->ev_cap > 0) {
-            const uint32_t c = sc->ev_cap;
+        // Events: into the scene's buffer, reduced at the end of this call,
+        // or, inside a gradient window, appended and reduced when the window
+        // ends. Without room for events every contribution goes to the
+        // gradient with atomics instead.
+        const uint32_t cap = sc->ev_window ? sc->ev_window_cap : ensure_event_buffer(sc, (uint64_t)npix * samples);
+        if (cap > 0) {
             W.ev_base = static_cast<char *>(sc->d_ev);
             W.ev_count = reinterpret_cast<unsigned long long *>(sc->d_evbins);
-            W.ev_cap = c;
-            E.ev_base = W.ev_base;
-            E.bins = sc->d_evbins;
-            E.ev_cap = c;
-            E.nbins = nbins;
-            E.shift = shift;
-            // Accumulation blocks of 2^14 events, or 2^12 for calls of fewer
-            // than 2^24 samples, whose few blocks would otherwise leave CUs
-            // idle (C1: 2^14 3.38 ms, 2^12 3.72 ms; a C3 view: fit step 42.1 ms
-            // at 2^14, 40.8 ms at 2^12).
-            E.acc_log2 = (uint64_t)npix * samples < (1ull << 24) ? 12u : 14u;
-            E.nbricks = k->scene->leafx[1];
-            E.lvoxc = k->P.lvoxc;
-            E.lvoxc3 = k->P.lvoxc3;
-            E.bce = reinterpret_cast<const float4 *>(sc->d_bce);
-            E.grad = adj->grad;
-            HIP_TRY(launch_fill(reinterpret_cast<float *>(sc->d_evbins), 2, 0.0f, s), "adjoint: clear");
+            W.ev_cap = cap;
+            if (!sc->ev_window) {
+                event_reduce_params(sc, cap, (uint64_t)npix * samples, adj->grad, E);
+                HIP_TRY(launch_fill(reinterpret_cast<float *>(sc->d_evbins), 2, 0.0f, s), "adjoint: clear");
+            }
         }
-    }
-    if (adj) {
         W.adj_l = adj->adj_l;
         W.adj_e = adj->adj_e;
         W.grad = adj->grad;
@@ -861,6 +884,7 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     HIP_TRY(launch_render(P, W, grid, mode, s, e0, e1), "render: launch");
     if (adj && copies > 1) HIP_TRY(launch_grad_sum(k->d_gradx, copies, gn, adj->grad, s), "adjoint: gradient sum");
     if (adj && E.ev_cap) HIP_TRY(launch_event_reduce(E, s), "adjoint: event reduction");
+    if (adj && sc->ev_window) sc->ev_window_samples += (uint64_t)npix * samples;
     return DRK_OK;
 }
 
@@ -1023,6 +1047,8 @@ int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
                        uint64_t seed, uint32_t sample_base, const float *adj_lights, const float *adj_env,
                        float *grad_fog) {
     if (!k || !adj_lights || !adj_env || !grad_fog) return set_error(DRK_EINVAL, "drk_kernel_adjoint: NULL argument");
+    if (k->scene->ev_window)
+        return set_error(DRK_EINVAL, "drk_kernel_adjoint: the scene's gradient window is open (use the _device forms)");
     if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
         return set_error(DRK_EINVAL, "drk_kernel_adjoint: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley,
                          k->tilex, k->tiley);
@@ -1089,6 +1115,8 @@ int drk_scene_footprint(const drk_scene *s, size_t *bytes) {
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes) {
     if (!k) return set_error(DRK_EINVAL, "drk_kernel_set_event_budget: NULL kernel");
     drk_scene *sc = k->scene;
+    if (sc->ev_window)
+        return set_error(DRK_EINVAL, "drk_kernel_set_event_budget: the scene's gradient window is open");
     sc->ev_budget = bytes == UINT64_MAX ? SIZE_MAX : (size_t)bytes;
     if (sc->d_ev || sc->d_evbins) (void)hipDeviceSynchronize();   // a launch may still use them
     if (sc->d_ev) (void)hipFree(sc->d_ev);
@@ -1096,6 +1124,31 @@ int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes) {
     sc->d_ev = nullptr;
     sc->d_evbins = nullptr;
     sc->ev_cap = 0;
+    return DRK_OK;
+}
+
+int drk_scene_events_begin(drk_scene *s, uint64_t samples, void *stream) {
+    if (!s) return set_error(DRK_EINVAL, "drk_scene_events_begin: NULL scene");
+    if (s->ev_window) return set_error(DRK_EINVAL, "drk_scene_events_begin: the scene's gradient window is already open");
+    if (samples == 0) return set_error(DRK_EINVAL, "drk_scene_events_begin: samples must be > 0");
+    const uint32_t cap = ensure_event_buffer(s, samples);
+    if (cap > 0)
+        HIP_TRY(launch_fill(reinterpret_cast<float *>(s->d_evbins), 2, 0.0f, (hipStream_t)stream),
+                "drk_scene_events_begin: clear");
+    s->ev_window = true;
+    s->ev_window_cap = cap;
+    s->ev_window_samples = 0;
+    return DRK_OK;
+}
+
+int drk_scene_events_end(drk_scene *s, float *d_grad_fog, void *stream) {
+    if (!s || !d_grad_fog) return set_error(DRK_EINVAL, "drk_scene_events_end: NULL argument");
+    if (!s->ev_window) return set_error(DRK_EINVAL, "drk_scene_events_end: no gradient window is open");
+    s->ev_window = false;
+    if (s->ev_window_cap == 0 || s->ev_window_samples == 0) return DRK_OK;
+    EventReduce E;
+    event_reduce_params(s, s->ev_window_cap, s->ev_window_samples, d_grad_fog, E);
+    HIP_TRY(launch_event_reduce(E, (hipStream_t)stream), "drk_scene_events_end: event reduction");
     return DRK_OK;
 }
 

@@ -3,7 +3,8 @@
 `DensityFit(fw)` renders the ground truth of a `configs.FitWorkload` from
 every view (targets, per-pixel means), and fits the FOG brick values of
 `fw.init` to them with an L2 loss on the images and Adam, differentiating
-each view's render with the path-replay adjoint (`autograd.render`). Brick
+the views' renders with the path-replay adjoint (`autograd.render_views`:
+one event reduction for all views of a step). Brick
 values are clamped to [0, majorant] after each step, as the adjoint keeps
 the majorants fixed.
 """
@@ -50,9 +51,10 @@ class DensityFit:
     def loss(self, rho, seed, spp=None, views=None):
         """Sum over views of the squared error of the per-pixel mean RGB."""
         spp = spp or self.fw.spp
+        idx = list(range(len(self.kernels)) if views is None else views)
+        imgs = AG.render_views([self.kernels[i] for i in idx], rho, spp, [seed * 1009 + i for i in idx])
         total = 0.0
-        for i in (range(len(self.kernels)) if views is None else views):
-            il, ie = AG.render(self.kernels[i], rho, samples=spp, seed=seed * 1009 + i)
+        for i, (il, ie) in zip(idx, imgs):
             img = (il + ie)[..., :3] / spp
             total = total + ((img - self.targets[i]) ** 2).sum()
         return total

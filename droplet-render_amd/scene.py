@@ -150,6 +150,20 @@ class Scene:
             ptr = C.c_void_p(self._ref.ctypes.data)
         L.check(L.lib().drk_scene_set_reference_fog(self.device(), ptr))
 
+    def events_begin(self, samples, stream=None):
+        """Opens the gradient window: the _device adjoint calls of this
+        scene's kernels append their events for one reduction at
+        events_end (`samples`: the pixel samples of all those calls)."""
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_scene_events_begin(self.device(), int(samples), s))
+
+    def events_end(self, grad, stream=None):
+        """Closes the gradient window: adds the reduction of its events to
+        `grad` (a CUDA tensor of leafx*lvoxc^3 floats)."""
+        assert grad.is_cuda and grad.dtype.itemsize == 4 and grad.numel() == self.leafx[1] * self.lvoxc3
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_scene_events_end(self.device(), C.c_void_p(grad.data_ptr()), s))
+
     def raise_majorants(self, factor=1.25, floor=None):
         """Multiplies every non-zero FOG majorant by `factor` (any value at or
         above the brick maximum keeps delta tracking unbiased); used to keep

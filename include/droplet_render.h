@@ -309,6 +309,23 @@ int drk_kernel_render_weighted_device(drk_kernel *k, uint32_t x0, uint32_t y0, u
  */
 int drk_kernel_set_event_budget(drk_kernel *k, uint64_t bytes);
 
+/*
+ * Gradient window (no reference counterpart): one event reduction for many
+ * adjoint calls, e.g. the views of one optimisation step. After
+ * drk_scene_events_begin, the _device adjoint entry points of the scene's
+ * kernels append their events to the scene's event buffer (sized for
+ * `samples` pixel samples in total: 16 slots each, within the budget above)
+ * instead of reducing them at the end of each call; drk_scene_events_end
+ * reduces all of them at once and adds the result to d_grad_fog. Events
+ * beyond the buffer go to each call's own d_grad_fog with atomics, so the
+ * gradient is the sum of what the calls and the end add (pass one gradient
+ * to all of them to get it in one place). The calls and the end must run in
+ * order (one stream, or synchronised by the caller). While the window is
+ * open the host adjoint and drk_kernel_set_event_budget are refused.
+ */
+int drk_scene_events_begin(drk_scene *scene, uint64_t samples, void *stream);
+int drk_scene_events_end(drk_scene *scene, float *d_grad_fog, void *stream);
+
 /* Device memory held by a kernel (result, partial-sum and gradient-copy
  * buffers as currently allocated) and by a scene (node records, child links,
  * FOG bricks, the adjoint's event buffers), in bytes. No reference

@@ -77,6 +77,14 @@ class BatchJob(C.Structure):
                 ("d_lights", C.c_void_p), ("d_env", C.c_void_p), ("d_sumsq_lights", C.c_void_p),
                 ("d_sumsq_env", C.c_void_p)]
 
+
+class ViewJob(C.Structure):
+    """drk_view_job: one call of a views launch (record forward or adjoint)."""
+    _fields_ = [("kernel", C.c_void_p), ("x0", C.c_uint32), ("y0", C.c_uint32), ("tilex", C.c_uint32),
+                ("tiley", C.c_uint32), ("samples", C.c_uint32), ("seed", C.c_uint64), ("sample_base", C.c_uint32),
+                ("d_lights", C.c_void_p), ("d_env", C.c_void_p), ("d_record", C.c_void_p),
+                ("record_bytes", C.c_size_t)]
+
 _lib = None
 
 _P = C.c_void_p
@@ -135,6 +143,8 @@ _SIGS = {
     "drk_batch_launch_graph": (C.c_int, [_P, _P]),
     "drk_batch_autotune": (C.c_int, [_P, _P, u32, C.c_char_p, u32]),
     "drk_batch_destroy": (None, [_P]),
+    "drk_views_render_record_device": (C.c_int, [C.POINTER(ViewJob), u32, _P]),
+    "drk_views_adjoint_record_device": (C.c_int, [C.POINTER(ViewJob), u32, _P, _P]),
     "drk_synth_cloud": (C.c_int, [u32, u32, u64, _F, u32, _F]),
     "drk_probe_philox": (C.c_int, [_U, _U, u32, _U]),
     "drk_probe_ray_stream": (C.c_int, [_U, _U, _U, u32, u32, u32, _F]),

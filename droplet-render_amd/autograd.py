@@ -8,6 +8,9 @@ With record=True (the default) the forward keeps each path's radiance and
 shadow-escape bits (40 bytes per pixel sample, drk_kernel_render_record_device)
 until backward, which then traces every path once instead of twice
 (drk_kernel_adjoint_record_device); record=False keeps no per-sample state.
+`render_views(kernels, density, samples, seeds)` differentiates the full-frame
+renders of several kernels of one scene (views) as one node: one launch for
+the forward of all views and one for their adjoints, with one event reduction.
 The octree's majorants stay fixed, as the adjoint assumes: keep the densities
 at or below them (`Scene.raise_majorants`).
 """
@@ -62,13 +65,15 @@ class _Render(torch.autograd.Function):
 
 
 class _RenderViews(torch.autograd.Function):
-    """Several full-frame renders of one scene (views) as one node: the
-    forward renders every view, the backward runs every view's adjoint into
-    one gradient inside the scene's gradient window, so their events are
-    reduced once."""
+    """Several full-frame renders of one scene (views) as one node. With the
+    path record, the forward renders every view in one launch and the
+    backward runs every view's adjoint in one launch with one event
+    reduction (drk_views_*); without it, the views render one by one and
+    their adjoints share the scene's gradient window (one reduction)."""
 
     @staticmethod
     def forward(ctx, density, kernels, samples, seeds, record):
+        from . import batch as B
         scene = kernels[0].scene
         assert all(k.scene is scene for k in kernels)
         assert density.is_cuda and density.dtype == torch.float32 and density.is_contiguous()
@@ -76,7 +81,7 @@ class _RenderViews(torch.autograd.Function):
         torch.cuda.current_stream(density.device).synchronize()
         scene.update_fog(density)
         stream = torch.cuda.current_stream(density.device)
-        outs, recs = [], []
+        outs, recs, jobs = [], [], []
         for k, seed in zip(kernels, seeds):
             n = k.tilex * k.tiley * 4
             out_l = torch.empty(n, dtype=torch.float32, device=density.device)
@@ -84,17 +89,20 @@ class _RenderViews(torch.autograd.Function):
             if record:
                 nb = k.record_bytes(0, 0, k.tilex, k.tiley, samples)
                 rec = torch.empty((nb + 15) // 16 * 4, dtype=torch.float32, device=density.device)
-                k.render_record_device(0, 0, k.tilex, k.tiley, samples, seed, 0, out_l, out_e, rec, stream=stream)
+                jobs.append((k, 0, 0, k.tilex, k.tiley, samples, seed, 0, out_l, out_e, rec))
                 recs.append(rec)
             else:
                 k.render_device(0, 0, k.tilex, k.tiley, samples, seed, 0, out_l, out_e, stream=stream)
             outs += [out_l.view(k.tiley, k.tilex, 4), out_e.view(k.tiley, k.tilex, 4)]
+        if record:
+            B.render_record_views(jobs, stream=stream)
         ctx.kernels, ctx.samples, ctx.seeds, ctx.recs = kernels, samples, seeds, recs if record else None
         ctx.save_for_backward(density)
         return tuple(outs)
 
     @staticmethod
     def backward(ctx, *grads):
+        from . import batch as B
         (density,) = ctx.saved_tensors
         kernels, samples = ctx.kernels, ctx.samples
         scene = kernels[0].scene
@@ -102,24 +110,28 @@ class _RenderViews(torch.autograd.Function):
         scene.update_fog(density)
         stream = torch.cuda.current_stream(density.device)
         grad = torch.zeros_like(density)
-        total = sum(k.tilex * k.tiley * samples for k in kernels)
-        scene.events_begin(total, stream=stream)
+        views = []
+        for i, (k, seed) in enumerate(zip(kernels, ctx.seeds)):
+            g_l, g_e = grads[2 * i], grads[2 * i + 1]
+            if g_l is None and g_e is None:
+                continue
+            shape = (k.tiley, k.tilex, 4)
+            g_l = torch.zeros(shape, device=density.device) if g_l is None else g_l.contiguous().float()
+            g_e = torch.zeros(shape, device=density.device) if g_e is None else g_e.contiguous().float()
+            views.append((i, k, seed, g_l, g_e))
+        if ctx.recs is not None:
+            if views:
+                jobs = [(k, 0, 0, k.tilex, k.tiley, samples, seed, 0, g_l, g_e, ctx.recs[i])
+                        for i, k, seed, g_l, g_e in views]
+                B.adjoint_record_views(jobs, grad, stream=stream)
+            ctx.recs = None
+            return grad, None, None, None, None
+        scene.events_begin(sum(k.tilex * k.tiley * samples for _, k, _, _, _ in views) or 1, stream=stream)
         try:
-            for i, (k, seed) in enumerate(zip(kernels, ctx.seeds)):
-                g_l, g_e = grads[2 * i], grads[2 * i + 1]
-                if g_l is None and g_e is None:
-                    continue
-                shape = (k.tiley, k.tilex, 4)
-                g_l = torch.zeros(shape, device=density.device) if g_l is None else g_l.contiguous().float()
-                g_e = torch.zeros(shape, device=density.device) if g_e is None else g_e.contiguous().float()
-                if ctx.recs is not None:
-                    k.adjoint_record_device(0, 0, k.tilex, k.tiley, samples, seed, 0, g_l, g_e, grad, ctx.recs[i],
-                                            stream=stream)
-                else:
-                    k.adjoint_device(0, 0, k.tilex, k.tiley, samples, seed, 0, g_l, g_e, grad, stream=stream)
+            for i, k, seed, g_l, g_e in views:
+                k.adjoint_device(0, 0, k.tilex, k.tiley, samples, seed, 0, g_l, g_e, grad, stream=stream)
         finally:
             scene.events_end(grad, stream=stream)
-        ctx.recs = None
         return grad, None, None, None, None
 
 

@@ -5,6 +5,12 @@ sample_base, out_lights, out_env)` tuples (outputs: CUDA float32 tensors of
 tilex*tiley*4) and wraps drk_batch_*: `launch()` enqueues the batched kernel,
 `launch_graph()` replays it as a captured hipGraph, `autotune()` picks the
 grid and the work-unit granularity by timing a sweep.
+
+`render_record_views(jobs)` and `adjoint_record_views(jobs, grad)` run the
+path-record forward and its adjoint of several calls in one persistent
+launch each (drk_views_*): jobs are `(kernel, x0, y0, tilex, tiley, samples,
+seed, sample_base, lights, env, record)` with CUDA float32 tensors (the sums,
+or their gradients for the adjoint) and the record tensor.
 """
 import ctypes as C
 
@@ -62,3 +68,30 @@ class Batch:
             self.destroy()
         except Exception:
             pass
+
+
+def _view_jobs(jobs):
+    arr = (L.ViewJob * len(jobs))()
+    for i, job in enumerate(jobs):
+        k, x0, y0, tx, ty, spp, seed, base, t_l, t_e, rec = job
+        for t in (t_l, t_e):
+            assert t.is_cuda and str(t.dtype) == "torch.float32" and t.is_contiguous()
+            assert t.numel() >= tx * ty * 4
+        assert rec.is_cuda and rec.is_contiguous()
+        arr[i] = L.ViewJob(k._h, int(x0), int(y0), int(tx), int(ty), int(spp), int(seed), int(base),
+                           t_l.data_ptr(), t_e.data_ptr(), rec.data_ptr(), rec.numel() * rec.element_size())
+    return arr
+
+
+def render_record_views(jobs, stream=None):
+    """Record forward of every job, one launch (same sums and records as
+    separate render_record_device calls)."""
+    L.check(L.lib().drk_views_render_record_device(_view_jobs(jobs), len(jobs), _stream_ptr(stream)))
+
+
+def adjoint_record_views(jobs, grad, stream=None):
+    """Adjoint of every job into `grad` (a CUDA float32 tensor), one launch
+    and one event reduction (none inside the scene's gradient window)."""
+    assert grad.is_cuda and str(grad.dtype) == "torch.float32" and grad.is_contiguous()
+    L.check(L.lib().drk_views_adjoint_record_device(_view_jobs(jobs), len(jobs), C.c_void_p(grad.data_ptr()),
+                                                    _stream_ptr(stream)))

@@ -135,6 +135,8 @@ struct drk_kernel {
     uint32_t round_limit = 0;       // watchdog: rounds per wave and launch (0: none)
     uint32_t adj_estimator = DRK_ADJ_SEGMENT;   // free-flight terms of the adjoint
     uint32_t *status = nullptr;     // counter block of the last launch (its status word at kStatusWord)
+    void *d_jobs = nullptr;         // job tables of views calls led by this kernel (RenderParams, WorkParams)
+    size_t jobs_bytes = 0;
     char *scratch = nullptr;        // this device's fixed scratch (allocation-free render), or null
     uint32_t chunk_override = 0;
     uint32_t max_chunks = (uint32_t)kMaxAutoChunks;   // default chunking: at most this many per pixel
@@ -625,6 +627,18 @@ static void rendered_extent(const drk_kernel *k, uint32_t x0, uint32_t y0, uint3
     ry = y0 < k->P.h ? std::min(tiley & ~1u, k->P.h - y0) : 0u;
 }
 
+// One job of a views call (several launches as one batched launch): what
+// render_impl plans for it instead of launching, and what the call shares
+// among its jobs (event slots, gradient copies).
+struct ViewsPlan {
+    uint32_t ev_cap = 0;          // event slots of the call (0: atomics only)
+    float *gradx = nullptr;       // per-XCD gradient copies (copies > 1)
+    uint32_t copies = 1;
+    bool planned = false;         // false: the job renders nothing (empty extent)
+    RenderParams P;
+    WorkParams W;
+};
+
 // Event buffer of the scene's adjoint calls: 40 bytes per slot (event +
 // sorted copy), sized to 16 events per sample of `samples` within the
 // budget (a multiple of 64 slots, which keeps every plane 16-byte aligned);
@@ -691,10 +705,35 @@ static void event_reduce_params(const drk_scene *sc, uint32_t cap, uint64_t samp
     E.grad = grad;
 }
 
+// Per-XCD gradient copies of an adjoint launch: one per XCD by default, while
+// they stay small next to HBM (<= 4 GiB).
+static uint32_t grad_copies_for(const drk_kernel *k, size_t gn) {
+    uint32_t copies = k->grad_copies;
+    while (copies > 1 && (double)copies * gn * sizeof(float) > 4.0 * (1ull << 30)) copies >>= 1;
+    return copies;
+}
+
+// The kernel's per-XCD gradient copies, grown to copies * gn floats and
+// cleared on stream s.
+static hipError_t clear_grad_copies(drk_kernel *k, uint32_t copies, size_t gn, hipStream_t s, float **out) {
+    const size_t need = (size_t)copies * gn * sizeof(float);
+    if (need > k->gradx_bytes) {
+        if (k->d_gradx) (void)hipFree(k->d_gradx);
+        k->d_gradx = nullptr;
+        k->gradx_bytes = 0;
+        hipError_t e = dev_malloc(&k->d_gradx, need);
+        if (e != hipSuccess) return e;
+        k->gradx_bytes = need;
+    }
+    *out = k->d_gradx;
+    return launch_fill(k->d_gradx, (size_t)copies * gn, 0.0f, s);
+}
+
 static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, float *d_l, float *d_e, float *d_sql, float *d_sqe,
                        hipStream_t s, bool count, bool shadow = false, const AdjointArgs *adj = nullptr,
-                       float *rec = nullptr, bool noalloc = false, bool weighted = false) {
+                       float *rec = nullptr, bool noalloc = false, bool weighted = false,
+                       ViewsPlan *vp = nullptr) {
     if (tilex == 0 || tiley == 0 || tilex > k->tilex || tiley > k->tiley)
         return set_error(DRK_EINVAL, "render: tile %ux%u exceeds the kernel's %ux%u", tilex, tiley, k->tilex, k->tiley);
     if (samples == 0) return set_error(DRK_EINVAL, "render: samples must be > 0");
@@ -788,12 +827,14 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         // or, inside a gradient window, appended and reduced when the window
         // ends. Without room for events every contribution goes to the
         // gradient with atomics instead.
-        const uint32_t cap = sc->ev_window ? sc->ev_window_cap : ensure_event_buffer(sc, (uint64_t)npix * samples);
+        const uint32_t cap = vp              ? vp->ev_cap
+                             : sc->ev_window ? sc->ev_window_cap
+                                             : ensure_event_buffer(sc, (uint64_t)npix * samples);
         if (cap > 0) {
             W.ev_base = static_cast<char *>(sc->d_ev);
             W.ev_count = reinterpret_cast<unsigned long long *>(sc->d_evbins);
             W.ev_cap = cap;
-            if (!sc->ev_window) {
+            if (!sc->ev_window && !vp) {
                 event_reduce_params(sc, cap, (uint64_t)npix * samples, adj->grad, E);
                 HIP_TRY(launch_fill(reinterpret_cast<float *>(sc->d_evbins), 2, 0.0f, s), "adjoint: clear");
             }
@@ -801,20 +842,13 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
         W.adj_l = adj->adj_l;
         W.adj_e = adj->adj_e;
         W.grad = adj->grad;
-        // Per-XCD copies while they stay small next to HBM (<= 4 GiB).
-        copies = k->grad_copies;
-        while (copies > 1 && (double)copies * gn * sizeof(float) > 4.0 * (1ull << 30)) copies >>= 1;
+        // Per-XCD copies of the gradient (the atomics' targets), added to
+        // grad after the launch.
+        copies = vp ? vp->copies : grad_copies_for(k, gn);
         if (copies > 1) {
-            size_t need = (size_t)copies * gn * sizeof(float);
-            if (need > k->gradx_bytes) {
-                if (k->d_gradx) (void)hipFree(k->d_gradx);
-                k->d_gradx = nullptr;
-                k->gradx_bytes = 0;
-                HIP_TRY(dev_malloc(&k->d_gradx, need), "adjoint: gradient copies");
-                k->gradx_bytes = need;
-            }
-            HIP_TRY(launch_fill(k->d_gradx, (size_t)copies * gn, 0.0f, s), "adjoint: clear");
-            W.grad = k->d_gradx;
+            float *gx = vp ? vp->gradx : nullptr;
+            if (!vp) HIP_TRY(clear_grad_copies(k, copies, gn, s, &gx), "adjoint: gradient copies");
+            W.grad = gx;
             W.grad_copies = copies;
             W.grad_stride = gn;
         }
@@ -840,8 +874,9 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
     }
     // Mailboxes of the escape service (forward modes; the allocation-free
     // entry computes escape radiance in place).
-    if (!noalloc && (mode == kModeForward || mode == kModeForwardSq || mode == kModeCount || mode == kModeWeighted ||
-                     mode == kModeRecord)) {
+    if (!noalloc && !vp &&
+        (mode == kModeForward || mode == kModeForwardSq || mode == kModeCount || mode == kModeWeighted ||
+         mode == kModeRecord)) {
         const size_t need = (size_t)grid * render_block_size() * 32;
         if (need > k->mail_bytes) {
             if (k->d_mail) (void)hipFree(k->d_mail);
@@ -866,6 +901,14 @@ static int render_impl(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, 
             k->cam_bytes = need;
         }
         W.cam = static_cast<float4 *>(k->d_cam);
+    }
+    if (vp) {
+        // Planned for a batched launch by the caller.
+        k->status = W.counter;
+        vp->P = P;
+        vp->W = W;
+        vp->planned = true;
+        return DRK_OK;
     }
     if (count) HIP_TRY(hipMemsetAsync(k->d_counts, 0, 16, s), "render: counters");
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1043,6 +1086,116 @@ int drk_kernel_adjoint_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, ui
                         d_record, stream, "drk_kernel_adjoint_record_device");
 }
 
+// Several record-forward or adjoint calls as one batched launch: each job is
+// planned by render_impl exactly as its own call would be (tile, chunks,
+// record, buffers of its kernel), the plans go to a job table on the device,
+// and one k_render_batch runs them all; then the per-job chunk reductions
+// (forward) or the shared gradient sum and event reduction (adjoint).
+static_assert(sizeof(RenderParams) % alignof(WorkParams) == 0, "the job table's WorkParams follow its RenderParams");
+static int views_impl(const drk_view_job *jobs, uint32_t nj, float *d_grad, void *stream, const char *fn) {
+    const bool adjoint = d_grad != nullptr;
+    if (!jobs || nj == 0) return set_error(DRK_EINVAL, "%s: no jobs", fn);
+    drk_kernel *k0 = jobs[0].kernel;
+    for (uint32_t j = 0; j < nj; ++j) {
+        const drk_view_job &J = jobs[j];
+        if (!J.kernel || !J.d_lights || !J.d_env) return set_error(DRK_EINVAL, "%s: job %u: NULL argument", fn, j);
+        for (uint32_t i = 0; i < j; ++i)
+            if (jobs[i].kernel == J.kernel) return set_error(DRK_EINVAL, "%s: jobs %u and %u share a kernel", fn, i, j);
+        if (J.kernel->P.scattevs > 64) return set_error(DRK_EINVAL, "%s: job %u: scattevs > 64 is not supported", fn, j);
+        int rc = check_record(J.kernel, J.x0, J.y0, J.tilex, J.tiley, J.samples, J.d_record, J.record_bytes, fn);
+        if (rc != DRK_OK) return rc;
+        if (adjoint) {
+            const drk_kernel *k = J.kernel;
+            if (k->scene != k0->scene) return set_error(DRK_EINVAL, "%s: job %u: the jobs' kernels need one scene", fn, j);
+            if (k->adj_estimator != k0->adj_estimator)
+                return set_error(DRK_EINVAL, "%s: job %u: the jobs' kernels need one adjoint estimator", fn, j);
+            if (k->depthcomp) return set_error(DRK_EINVAL, "%s: depth compositing is not differentiated", fn);
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;   // NULL: the default (null) stream, as in HIP
+    drk_scene *sc = k0->scene;
+    if (adjoint) {
+        if (sc->leafx[1] == 0) return set_error(DRK_EINVAL, "%s: scene has no FOG bricks", fn);
+        if (sc->leafx[0] != 0) return set_error(DRK_EINVAL, "%s: scenes with SDF leaves are not differentiated", fn);
+    }
+    const int mode = !adjoint ? kModeRecord
+                     : k0->adj_estimator == DRK_ADJ_NULL_COLLISION ? kModeAdjointNull : kModeAdjoint;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nj; ++j) {
+        uint32_t rx, ry;
+        rendered_extent(jobs[j].kernel, jobs[j].x0, jobs[j].y0, jobs[j].tilex, jobs[j].tiley, rx, ry);
+        total += (uint64_t)rx * ry * jobs[j].samples;
+    }
+    ViewsPlan vp;
+    EventReduce E;
+    std::memset(&E, 0, sizeof(E));
+    const size_t gn = (size_t)sc->leafx[1] * k0->P.lvoxc3;
+    if (adjoint && total) {
+        vp.ev_cap = sc->ev_window ? sc->ev_window_cap : ensure_event_buffer(sc, total);
+        if (vp.ev_cap && !sc->ev_window) {
+            event_reduce_params(sc, vp.ev_cap, total, d_grad, E);
+            HIP_TRY(launch_fill(reinterpret_cast<float *>(sc->d_evbins), 2, 0.0f, s), "views: clear");
+        }
+        vp.copies = grad_copies_for(k0, gn);
+        if (vp.copies > 1) HIP_TRY(clear_grad_copies(k0, vp.copies, gn, s, &vp.gradx), "views: gradient copies");
+    }
+    std::vector<RenderParams> hP;
+    std::vector<WorkParams> hW;
+    for (uint32_t j = 0; j < nj; ++j) {
+        const drk_view_job &J = jobs[j];
+        AdjointArgs A;
+        A.adj_l = J.d_lights;
+        A.adj_e = J.d_env;
+        A.grad = d_grad;
+        vp.planned = false;
+        int rc = render_impl(J.kernel, J.x0, J.y0, J.tilex, J.tiley, J.samples, J.seed, J.sample_base,
+                             adjoint ? nullptr : J.d_lights, adjoint ? nullptr : J.d_env, nullptr, nullptr, s, false,
+                             false, adjoint ? &A : nullptr, static_cast<float *>(J.d_record), false, false, &vp);
+        if (rc != DRK_OK) return rc;
+        if (vp.planned) {
+            hP.push_back(vp.P);
+            hW.push_back(vp.W);
+        }
+    }
+    const uint32_t n = (uint32_t)hP.size();
+    if (n == 0) return DRK_OK;
+    const size_t need = (size_t)n * (sizeof(RenderParams) + sizeof(WorkParams));
+    if (need > k0->jobs_bytes) {
+        if (k0->d_jobs) (void)hipFree(k0->d_jobs);
+        k0->d_jobs = nullptr;
+        k0->jobs_bytes = 0;
+        HIP_TRY(dev_malloc(&k0->d_jobs, need), "views: job table");
+        k0->jobs_bytes = need;
+    }
+    RenderParams *dP = static_cast<RenderParams *>(k0->d_jobs);
+    WorkParams *dW = reinterpret_cast<WorkParams *>(dP + n);
+    uint32_t cam_pix = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        HIP_TRY(launch_job_setup(hP[j], hW[j], dP + j, dW + j, s), "views: job setup");
+        if (hW[j].cam) cam_pix = std::max(cam_pix, hW[j].npix);
+    }
+    const uint32_t grid = k0->grid_override ? k0->grid_override
+                                            : k0->cu_count * (uint32_t)std::max(1, render_batch_blocks_per_cu(mode));
+    HIP_TRY(launch_render_batch(dP, dW, n, nullptr, 0, grid, mode, cam_pix, s), "views: launch");
+    if (!adjoint) {
+        for (uint32_t j = 0; j < n; ++j) HIP_TRY(launch_reduce(hW[j], hP[j].rx, hP[j].tilex, 2u, s), "views: reduce");
+        return DRK_OK;
+    }
+    if (vp.copies > 1) HIP_TRY(launch_grad_sum(vp.gradx, vp.copies, gn, d_grad, s), "views: gradient sum");
+    if (E.ev_cap) HIP_TRY(launch_event_reduce(E, s), "views: event reduction");
+    if (sc->ev_window) sc->ev_window_samples += total;
+    return DRK_OK;
+}
+
+int drk_views_render_record_device(const drk_view_job *jobs, uint32_t njobs, void *stream) {
+    return views_impl(jobs, njobs, nullptr, stream, "drk_views_render_record_device");
+}
+
+int drk_views_adjoint_record_device(const drk_view_job *jobs, uint32_t njobs, float *d_grad_fog, void *stream) {
+    if (!d_grad_fog) return set_error(DRK_EINVAL, "drk_views_adjoint_record_device: NULL gradient");
+    return views_impl(jobs, njobs, d_grad_fog, stream, "drk_views_adjoint_record_device");
+}
+
 int drk_kernel_adjoint(drk_kernel *k, uint32_t x0, uint32_t y0, uint32_t tilex, uint32_t tiley, uint32_t samples,
                        uint64_t seed, uint32_t sample_base, const float *adj_lights, const float *adj_env,
                        float *grad_fog) {
@@ -1209,6 +1362,7 @@ void drk_kernel_destroy(drk_kernel *k) {
     if (k->d_kc) (void)hipFree(k->d_kc);
     if (k->d_gradx) (void)hipFree(k->d_gradx);
     if (k->d_depth) (void)hipFree(k->d_depth);
+    if (k->d_jobs) (void)hipFree(k->d_jobs);
     for (auto &pr : k->ev_used) k->ev_free.push_back(pr);
     for (auto &pr : k->ev_free) {
         (void)hipEventDestroy(pr.first);

@@ -1289,9 +1289,16 @@ int render_blocks_per_cu(int mode) {
 
 int render_batch_blocks_per_cu(int mode) {
     int nb = 0;
-    hipError_t e = mode == kModeForwardSq
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeForwardSq>, kBlock, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeForward>, kBlock, 0);
+    hipError_t e;
+    switch (mode) {
+    case kModeForwardSq: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeForwardSq>, kBlock, 0); break;
+    case kModeRecord: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeRecord>, kBlock, 0); break;
+    case kModeAdjoint: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeAdjoint>, kBlock, 0); break;
+    case kModeAdjointNull:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeAdjointNull>, kBlock, 0);
+        break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<kModeForward>, kBlock, 0); break;
+    }
     if (e != hipSuccess || nb <= 0) {
         (void)hipGetLastError();
         return 1;
@@ -1302,11 +1309,30 @@ int render_batch_blocks_per_cu(int mode) {
 hipError_t launch_render_batch(const RenderParams *dP, const WorkParams *dW, uint32_t njobs, uint32_t *counters,
                                uint32_t ncounters, uint32_t grid, int mode, uint32_t cam_pix, hipStream_t s) {
     if (njobs == 0) return hipSuccess;
-    k_fill_u32<<<(ncounters + 255) / 256, 256, 0, s>>>(counters, ncounters, 0u);
+    if (counters) k_fill_u32<<<(ncounters + 255) / 256, 256, 0, s>>>(counters, ncounters, 0u);
     for (uint32_t j0 = 0; cam_pix && j0 < njobs; j0 += 65535u)
         k_camera_rays_batch<<<dim3((cam_pix + 255) / 256, min(65535u, njobs - j0)), 256, 0, s>>>(dP + j0, dW + j0);
-    if (mode == kModeForwardSq) k_render_batch<kModeForwardSq><<<grid, kBlock, 0, s>>>(dP, dW, njobs);
-    else k_render_batch<kModeForward><<<grid, kBlock, 0, s>>>(dP, dW, njobs);
+    switch (mode) {
+    case kModeForwardSq: k_render_batch<kModeForwardSq><<<grid, kBlock, 0, s>>>(dP, dW, njobs); break;
+    case kModeRecord: k_render_batch<kModeRecord><<<grid, kBlock, 0, s>>>(dP, dW, njobs); break;
+    case kModeAdjoint: k_render_batch<kModeAdjoint><<<grid, kBlock, 0, s>>>(dP, dW, njobs); break;
+    case kModeAdjointNull: k_render_batch<kModeAdjointNull><<<grid, kBlock, 0, s>>>(dP, dW, njobs); break;
+    default: k_render_batch<kModeForward><<<grid, kBlock, 0, s>>>(dP, dW, njobs); break;
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_job_setup(RenderParams P, WorkParams W, RenderParams *dP, WorkParams *dW) {
+    for (uint32_t i = threadIdx.x; i < kCounterWords; i += blockDim.x) W.counter[i] = 0u;
+    if (threadIdx.x == 0) {
+        *dP = P;
+        *dW = W;
+    }
+}
+
+hipError_t launch_job_setup(const RenderParams &P, const WorkParams &W, RenderParams *dP, WorkParams *dW,
+                            hipStream_t s) {
+    k_job_setup<<<1, 256, 0, s>>>(P, W, dP, dW);
     return hipGetLastError();
 }
 

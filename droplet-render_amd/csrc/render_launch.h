@@ -180,13 +180,19 @@ constexpr size_t kScratchPartialOffset = 4096;
 constexpr uint32_t kScratchDevices = 64;
 hipError_t scratch_address(void **ptr);
 static_assert(kLaunchBlockBytes <= kScratchPartialOffset, "launch block fits before the partials");
-// Batched forward launch over njobs (P, W) pairs in device memory; counters
-// (ncounters words, every job's band counters) are cleared first.
+// Batched launch over njobs (P, W) pairs in device memory (modes: forward,
+// forward with squares, forward with the path record, the two adjoints);
+// counters (ncounters words, every job's band counters; null: the caller
+// cleared them) are cleared first.
 int render_batch_blocks_per_cu(int mode);
 // cam_pix: the largest job's pixel count when the jobs' camera rays are to be
 // computed first (their W.cam), else 0.
 hipError_t launch_render_batch(const RenderParams *dP, const WorkParams *dW, uint32_t njobs, uint32_t *counters,
                                uint32_t ncounters, uint32_t grid, int mode, uint32_t cam_pix, hipStream_t s);
+// Stores one job's parameters at *dP and *dW and clears its counter block
+// (W.counter), as kernels (no copies from host memory, nothing to keep alive).
+hipError_t launch_job_setup(const RenderParams &P, const WorkParams &W, RenderParams *dP, WorkParams *dW,
+                            hipStream_t s);
 // Chunk-order sum of a launch's partials into its outputs (no-op without partials).
 hipError_t launch_reduce(const WorkParams &W, uint32_t rx, uint32_t tilex, uint32_t buffers, hipStream_t s);
 // out[i] += sum over c < copies of part[c * n + i].

@@ -378,6 +378,36 @@ int drk_batch_autotune(drk_batch *b, void *stream, uint32_t reps, char *report, 
 void drk_batch_destroy(drk_batch *b);
 
 /*
+ * Views: the path-record forward and its adjoint for several render calls
+ * (views, tiles) in one persistent launch each (no reference counterpart).
+ * drk_views_render_record_device(jobs, n) gives every job the sums and
+ * path record of drk_kernel_render_record_device with the job's arguments
+ * (d_lights, d_env: the outputs), bit for bit; drk_views_adjoint_record_device
+ * (jobs, n, grad) adds to d_grad_fog what drk_kernel_adjoint_record_device
+ * would for every job (d_lights, d_env: the job's dL/d(sums), read; the
+ * same gradient up to the order of float additions), with one event
+ * reduction for all jobs (or none, inside the scene's gradient window). Each
+ * kernel at most once per call; the adjoint's kernels share one scene and
+ * adjoint estimator. A separate launch per call drains its last paths alone
+ * at the end; a views launch moves each workgroup on to the next job
+ * instead. Every job uses its kernel's buffers (counters, camera rays,
+ * partial sums) and the first job's kernel holds the job tables, so calls
+ * that share kernels must not overlap in time. The round limit applies per
+ * job (drk_kernel_limit_hit of each job's kernel).
+ */
+typedef struct drk_view_job {
+    drk_kernel *kernel;
+    uint32_t x0, y0, tilex, tiley, samples;
+    uint64_t seed;
+    uint32_t sample_base;
+    float *d_lights, *d_env;   /* tilex*tiley float4 each: the sums (render) or their gradients (adjoint) */
+    void *d_record;            /* the job's path record, 16-byte aligned */
+    size_t record_bytes;
+} drk_view_job;
+int drk_views_render_record_device(const drk_view_job *jobs, uint32_t njobs, void *stream);
+int drk_views_adjoint_record_device(const drk_view_job *jobs, uint32_t njobs, float *d_grad_fog, void *stream);
+
+/*
  * Extensions with no reference counterpart.
  *
  * drk_kernel_set_tuning: fixes the work-unit chunk (samples per unit, every

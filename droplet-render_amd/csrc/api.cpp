@@ -1092,6 +1092,7 @@ int drk_kernel_adjoint_record_device(drk_kernel *k, uint32_t x0, uint32_t y0, ui
 // and one k_render_batch runs them all; then the per-job chunk reductions
 // (forward) or the shared gradient sum and event reduction (adjoint).
 static_assert(sizeof(RenderParams) % alignof(WorkParams) == 0, "the job table's WorkParams follow its RenderParams");
+static int views_run(const drk_view_job *jobs, uint32_t nj, float *d_grad, uint64_t total, hipStream_t s);
 static int views_impl(const drk_view_job *jobs, uint32_t nj, float *d_grad, void *stream, const char *fn) {
     const bool adjoint = d_grad != nullptr;
     if (!jobs || nj == 0) return set_error(DRK_EINVAL, "%s: no jobs", fn);
@@ -1118,14 +1119,44 @@ static int views_impl(const drk_view_job *jobs, uint32_t nj, float *d_grad, void
         if (sc->leafx[1] == 0) return set_error(DRK_EINVAL, "%s: scene has no FOG bricks", fn);
         if (sc->leafx[0] != 0) return set_error(DRK_EINVAL, "%s: scenes with SDF leaves are not differentiated", fn);
     }
-    const int mode = !adjoint ? kModeRecord
-                     : k0->adj_estimator == DRK_ADJ_NULL_COLLISION ? kModeAdjointNull : kModeAdjoint;
-    uint64_t total = 0;
-    for (uint32_t j = 0; j < nj; ++j) {
+    auto job_samples = [&](uint32_t j) {
         uint32_t rx, ry;
         rendered_extent(jobs[j].kernel, jobs[j].x0, jobs[j].y0, jobs[j].tilex, jobs[j].tiley, rx, ry);
-        total += (uint64_t)rx * ry * jobs[j].samples;
+        return (uint64_t)rx * ry * jobs[j].samples;
+    };
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < nj; ++j) total += job_samples(j);
+    if (adjoint && !sc->ev_window && nj > 1 && total) {
+        // Groups of consecutive jobs whose events (16 slots per sample, as a
+        // single call sizes them) fit the event buffer, one launch and one
+        // reduction per group: events beyond the buffer would go to float
+        // atomics, many times slower (C4's 8 views in one launch with a
+        // quarter of the free memory for events: 557 ms; in groups of 3:
+        // under 60 ms).
+        const uint64_t cap = ensure_event_buffer(sc, total);
+        if (cap > 0 && cap < 16 * total) {
+            for (uint32_t j0 = 0; j0 < nj;) {
+                uint32_t j1 = j0 + 1;
+                uint64_t g = job_samples(j0);
+                while (j1 < nj && 16 * (g + job_samples(j1)) <= cap) g += job_samples(j1++);
+                int rc = views_run(jobs + j0, j1 - j0, d_grad, g, s);
+                if (rc != DRK_OK) return rc;
+                j0 = j1;
+            }
+            return DRK_OK;
+        }
     }
+    return views_run(jobs, nj, d_grad, total, s);
+}
+
+// One batched launch of views jobs (validated by views_impl) that render
+// `total` pixel samples, with its reductions.
+static int views_run(const drk_view_job *jobs, uint32_t nj, float *d_grad, uint64_t total, hipStream_t s) {
+    const bool adjoint = d_grad != nullptr;
+    drk_kernel *k0 = jobs[0].kernel;
+    drk_scene *sc = k0->scene;
+    const int mode = !adjoint ? kModeRecord
+                     : k0->adj_estimator == DRK_ADJ_NULL_COLLISION ? kModeAdjointNull : kModeAdjoint;
     ViewsPlan vp;
     EventReduce E;
     std::memset(&E, 0, sizeof(E));

@@ -386,7 +386,10 @@ void drk_batch_destroy(drk_batch *b);
  * (jobs, n, grad) adds to d_grad_fog what drk_kernel_adjoint_record_device
  * would for every job (d_lights, d_env: the job's dL/d(sums), read; the
  * same gradient up to the order of float additions), with one event
- * reduction for all jobs (or none, inside the scene's gradient window). Each
+ * reduction for all jobs (none inside the scene's gradient window); jobs
+ * whose events (16 slots per pixel sample) would not fit the event buffer
+ * within its budget run as consecutive groups that fit, one launch and one
+ * reduction each. Each
  * kernel at most once per call; the adjoint's kernels share one scene and
  * adjoint estimator. A separate launch per call drains its last paths alone
  * at the end; a views launch moves each workgroup on to the next job

@@ -83,6 +83,7 @@ def test_c4_full_size(pkg, oracle, gpu):
     adj_l = torch.rand(n, generator=g).cuda()
     adj_e = torch.rand(n, generator=g).cuda()
     grad = torch.zeros(wl.scene.leafx[1] * wl.scene.lvoxc3, device="cuda")
+    sep = []
     for i, k in enumerate(ks):
         rl, re = torch.empty(n, device="cuda"), torch.empty(n, device="cuda")
         k.render_record_device(0, 0, wl.w, wl.h, wl.spp, 900 + i, 0, rl, re, rec)
@@ -91,9 +92,25 @@ def test_c4_full_size(pkg, oracle, gpu):
         assert torch.equal(rl.view(-1, 4)[:, 3], outs[i][0].view(-1, 4)[:, 3])
         assert torch.allclose(rl, outs[i][0], rtol=1e-5, atol=1e-6)
         k.adjoint_record_device(0, 0, wl.w, wl.h, wl.spp, 900 + i, 0, adj_l, adj_e, grad, rec)
+        sep.append((rl, re))
     torch.cuda.synchronize()
     gr = grad.cpu().numpy()
     assert np.isfinite(gr).all() and np.abs(gr).max() > 0
+    # the same 8 record forwards and adjoints as two views launches
+    del rec
+    recs = [torch.empty(rb // 4, device="cuda") for _ in ks]
+    vo = [(torch.empty(n, device="cuda"), torch.empty(n, device="cuda")) for _ in ks]
+    pkg.batch.render_record_views([(k, 0, 0, wl.w, wl.h, wl.spp, 900 + i, 0, vo[i][0], vo[i][1], recs[i])
+                                   for i, k in enumerate(ks)])
+    grad2 = torch.zeros_like(grad)
+    pkg.batch.adjoint_record_views([(k, 0, 0, wl.w, wl.h, wl.spp, 900 + i, 0, adj_l, adj_e, recs[i])
+                                    for i, k in enumerate(ks)], grad2)
+    torch.cuda.synchronize()
+    for (a, b_), (c, d) in zip(vo, sep):
+        assert torch.equal(a, c) and torch.equal(b_, d)
+    g2 = grad2.cpu().numpy()
+    assert np.abs(g2.astype(np.float64) - gr).max() <= 1e-5 * np.abs(gr).max()
+    del recs
     fp = wl.scene.footprint() + sum(k.footprint() for k in ks)
     print("c4 coverage", np.round(cover, 4), "footprint GB %.2f" % (fp / 1e9))
     # one 48x48 tile around the cloud of view 0 against the oracle (same keys)

@@ -60,6 +60,14 @@ def test_host_argument_errors(pkg):
     assert L.drk_kernel_create(None, None, None, None, None, 1, 1.0, 0.0, 8, 8, 8, 8, 0, C.byref(h)) == pkg._lib.DRK_EINVAL
     assert L.drk_kernel_render(None, 0, 0, 8, 8, 1) == pkg._lib.DRK_EINVAL
     assert L.drk_kernel_phb(None, 0) is None or not bool(L.drk_kernel_phb(None, 0))
+    # views and gradient-window entry points: argument errors before any device work
+    assert L.drk_views_render_record_device(None, 0, None) == pkg._lib.DRK_EINVAL
+    jobs = (pkg._lib.ViewJob * 1)()
+    assert L.drk_views_render_record_device(jobs, 1, None) == pkg._lib.DRK_EINVAL   # NULL kernel
+    assert L.drk_views_adjoint_record_device(jobs, 1, None, None) == pkg._lib.DRK_EINVAL
+    assert L.drk_scene_events_begin(None, 1, None) == pkg._lib.DRK_EINVAL
+    assert L.drk_scene_events_end(None, None, None) == pkg._lib.DRK_EINVAL
+    assert C.sizeof(pkg._lib.ViewJob) == 80 and pkg._lib.ViewJob.d_lights.offset == 48
 
 
 def test_scene_create_rejects_bad_octrees_on_host(pkg):

@@ -559,13 +559,18 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         const F3 pL = mk3(l.x + b.x * lc.x, l.y + b.y * lc.y, l.z + b.z * lc.z);
         const F3 pE = mk3(v.x + b.x * le.x, v.y + b.y * le.y, v.z + b.z * le.z);
         if (kRec) {
-            // The path record holds the path's radiance as well (index in
-            // the mailbox's fourth word).
+            // The path record's radiance and alpha (index in the mailbox's
+            // fourth word): the unit's sums were zero before this path, so
+            // they hold the path's radiance and alpha exactly, and with the
+            // escape added they are the entry the lane would have written.
+            // The lane writes the entry's fourth word of the second half.
             float4 *R = reinterpret_cast<float4 *>(W.rec);
             const size_t n = (size_t)W.npix * P.samples, ri = __float_as_uint(b.w);
-            const float4 r0 = R[ri], r1 = R[n + ri];
-            R[ri] = make_float4(r0.x + b.x * lc.x, r0.y + b.y * lc.y, r0.z + b.z * lc.z, r0.w);
-            R[n + ri] = make_float4(r1.x + b.x * le.x, r1.y + b.y * le.y, r1.z + b.z * le.z, r1.w);
+            R[ri] = make_float4(pL.x, pL.y, pL.z, l.w);
+            float *e = reinterpret_cast<float *>(R + n + ri);
+            e[0] = pE.x;
+            e[1] = pE.y;
+            e[2] = pE.z;
         }
         *oL = make_float4(0.0f + pL.x, 0.0f + pL.y, 0.0f + pL.z, l.w);
         *oE = make_float4(0.0f + pE.x, 0.0f + pE.y, 0.0f + pE.z, v.w);
@@ -1068,8 +1073,15 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                     const size_t n = (size_t)W.npix * P.samples;
                     const size_t ri = (size_t)s_cur * W.npix + pix_local;
                     float4 *R = reinterpret_cast<float4 *>(W.rec);
-                    R[ri] = make_float4(pathL.x, pathL.y, pathL.z, alpha);
-                    R[n + ri] = make_float4(pathE.x, pathE.y, pathE.z, __int_as_float(last_c));
+                    if (kEscSvc && esc_defer) {
+                        // The escape service writes the entry's radiance and
+                        // alpha (from the unit's sums, which are this path's);
+                        // the lane writes where its last contribution arose.
+                        reinterpret_cast<float *>(R + n + ri)[3] = __int_as_float(last_c);
+                    } else {
+                        R[ri] = make_float4(pathL.x, pathL.y, pathL.z, alpha);
+                        R[n + ri] = make_float4(pathE.x, pathE.y, pathE.z, __int_as_float(last_c));
+                    }
                     reinterpret_cast<uint2 *>(R + 2 * n)[ri] = make_uint2((uint32_t)vis, (uint32_t)(vis >> 32));
                     vis = 0;
                 }

@@ -694,9 +694,9 @@ static void event_reduce_params(const drk_scene *sc, uint32_t cap, uint64_t samp
     E.nbins = nbins;
     E.shift = shift;
     // Accumulation blocks of 2^14 events, or 2^12 for calls of fewer
-    // than 2^24 samples, whose few blocks would otherwise leave CUs
-    // idle (C1: 2^14 3.38 ms, 2^12 3.72 ms; a C3 view: fit step 42.1 ms
-    // at 2^14, 40.8 ms at 2^12).
+    // than 2^24 samples, whose few blocks would otherwise leave CUs idle
+    // (256^3 frame: 2^14 3.38 ms, 2^12 3.72 ms; sixteen 256^2 fitting views
+    // reduced one by one: step 42.1 ms at 2^14, 40.8 ms at 2^12).
     E.acc_log2 = samples < (1ull << 24) ? 12u : 14u;
     E.nbricks = sc->leafx[1];
     E.lvoxc = sc->lvoxc;
@@ -1130,9 +1130,9 @@ static int views_impl(const drk_view_job *jobs, uint32_t nj, float *d_grad, void
         // Groups of consecutive jobs whose events (16 slots per sample, as a
         // single call sizes them) fit the event buffer, one launch and one
         // reduction per group: events beyond the buffer would go to float
-        // atomics, many times slower (C4's 8 views in one launch with a
-        // quarter of the free memory for events: 557 ms; in groups of 3:
-        // under 60 ms).
+        // atomics, many times slower (a 512^3 cloud's 8 full-HD views in one
+        // launch with a quarter of the free memory for events: 557 ms; in
+        // groups of 3: 50.5 ms).
         const uint64_t cap = ensure_event_buffer(sc, total);
         if (cap > 0 && cap < 16 * total) {
             for (uint32_t j0 = 0; j0 < nj;) {

@@ -51,7 +51,7 @@ namespace drk {
 
 // Workgroups of eight waves, two per CU (four waves per SIMD): the
 // workgroup services draw their batches from twice the lanes of a
-// four-wave workgroup (C1 25.5 -> 25.3 ms, 25.1 with a scattering batch of
+// four-wave workgroup (256^3 frame 25.5 -> 25.3 ms, 25.1 with a scattering batch of
 // 48; sixteen waves per workgroup: 25.4 ms).
 constexpr int kBlock = 512;
 
@@ -74,7 +74,7 @@ enum : uint32_t { ST_WORK = 0, ST_PATH = 1, ST_RAY = 2, ST_TRAV = 3, ST_LEAF = 4
 #endif
 // The leaf-grid walk takes one cell step per round (grid_step): a step
 // costs far less than a Revelles node visit, and a longer loop only holds
-// the wave's other lanes (C1 with 1/2/3/4 steps per round:
+// the wave's other lanes (256^3 frame with 1/2/3/4 steps per round:
 // 29.5/31.2/32.8/33.6 ms).
 #ifndef DRK_UNIT_RESERVE
 #define DRK_UNIT_RESERVE 32
@@ -1040,7 +1040,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
                         // Blocked in the forward pass: no contribution, no
                         // gradient and no random numbers, so the continuation
                         // starts at once instead of after an empty shadow ray
-                        // (C1 fwd+adjoint 100.5 -> 90.9 ms).
+                        // (256^3 frame fwd+adjoint 100.5 -> 90.9 ms).
                         hit = true;
                         path_done = shadow_end();
                     }
@@ -1143,7 +1143,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
             }
             // Jobs are queued by the ray-end batch, so a wave looks for a
             // batch to serve only in rounds where it ran one (or when it
-            // cannot progress otherwise): C1 25.8 -> 25.5 ms.
+            // cannot progress otherwise): 256^3 frame 25.8 -> 25.5 ms.
             if (stalled || run_end) serve(stalled);
         }
     }
@@ -1166,7 +1166,7 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
 template <int kMode>
 // Four waves per SIMD (at most 128 VGPRs; the root interval and the
 // continuation state wait in LDS so that nothing spills): the fourth wave
-// hides more of the traversal's dependent loads (C1: 52.9 -> 46.9 ms).
+// hides more of the traversal's dependent loads (256^3 frame: 52.9 -> 46.9 ms).
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render(
     const LaunchParams *__restrict__ L) {
     __shared__ RenderLds S;
@@ -1176,7 +1176,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 
 // The adjoint, like the forward, at four waves per SIMD: its image gradient
 // is re-read from the tiles where needed, not kept in registers, so it fits
-// 128 VGPRs without spills (C1 forward+adjoint step 132.8 -> 124.3 ms).
+// 128 VGPRs without spills (256^3 frame forward+adjoint step 132.8 -> 124.3 ms).
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_render_adjoint(
     const LaunchParams *__restrict__ L) {
     __shared__ RenderLds S;
@@ -1512,7 +1512,7 @@ __device__ __forceinline__ void block_exscan(uint32_t *a, uint32_t n, uint32_t *
 // counting sort of the tile), so that consecutive lanes store consecutive
 // slots of a bin's run. Stored straight from their lanes (one cache line per
 // lane), the 16-byte events kept the texture addresser busy 80% of the
-// kernel's time (C1: 9.4 ms; through the tiles 7.2 ms; tiles of 3072 or 6144
+// kernel's time (256^3 frame: 9.4 ms; through the tiles 7.2 ms; tiles of 3072 or 6144
 // events measured no faster). The scatter also gathers each bin's largest
 // |weight| (an LDS maximum per event, one global maximum per bin and block)
 // so that the accumulation needs no pass of its own for the fixed-point
@@ -1689,7 +1689,7 @@ bool event_bins(uint32_t nbricks, uint32_t lvoxc3, uint32_t &shift, uint32_t &nb
     while (((uint64_t)nbricks + (1ull << shift) - 1) >> shift > kEvMaxBins) ++shift;
     // Up to 8 bricks per bin while a bin still fits the accumulation LDS:
     // fewer, fuller bins let the scatter write the sorted copy in longer runs
-    // (C1: 13.9 -> 11.1 ms), at a little more clearing and flushing per chunk.
+    // (256^3 frame: 13.9 -> 11.1 ms), at a little more clearing and flushing per chunk.
     while (shift < 3 && ((uint64_t)lvoxc3 * sizeof(unsigned long long) << (shift + 1)) <= kEvAccumLds) ++shift;
     nbins = (uint32_t)(((uint64_t)nbricks + (1ull << shift) - 1) >> shift);
     return nbricks > 0 && ((uint64_t)lvoxc3 * sizeof(unsigned long long) << shift) <= kEvAccumLds;

@@ -120,17 +120,19 @@ def _record_jobs(ks, spp, tiles=None):
         x0, y0, tx, ty = tiles[i] if tiles else (0, 0, k.tilex, k.tiley)
         nb = k.record_bytes(x0, y0, tx, ty, spp)
         o = (torch.full((tx * ty * 4,), -1.0, device="cuda"), torch.full((tx * ty * 4,), -1.0, device="cuda"),
-             torch.zeros((nb + 15) // 16 * 4, device="cuda"))
+             torch.zeros(max(4, (nb + 15) // 16 * 4), device="cuda"))
         jobs.append((k, x0, y0, tx, ty, spp, 100 + i, 3, o[0], o[1], o[2]))
         outs.append(o)
     return jobs, outs
 
 
-@pytest.mark.parametrize("tiles", [None, [(0, 0, 40, 40), (8, 6, 21, 17), (30, 30, 16, 16)]])
+@pytest.mark.parametrize("tiles", [None, [(0, 0, 40, 40), (8, 6, 21, 17), (30, 30, 16, 16)],
+                                   [(40, 0, 16, 16), (2, 2, 12, 10), (0, 38, 40, 8)]])
 def test_views_record_forward_equals_separate_calls(pkg, gpu, tiles):
     """Sums and path records of one views launch equal the separate
-    drk_kernel_render_record_device calls bit for bit (full frames, and
-    odd and overhanging tiles)."""
+    drk_kernel_render_record_device calls bit for bit (full frames; odd and
+    overhanging tiles; a tile wholly outside the image, which renders
+    nothing and is zeroed)."""
     import torch
     from droplet_render_amd import batch as B
     wl, ks, gs = _views(pkg)
@@ -145,7 +147,9 @@ def test_views_record_forward_equals_separate_calls(pkg, gpu, tiles):
             for name, a, b in zip(("lights", "env", "record"), o, r):
                 d = torch.nonzero(a.view(torch.int32) != b.view(torch.int32)).flatten()
                 assert d.numel() == 0, (v, name, d[:6].tolist(), a[d[:6]].tolist(), b[d[:6]].tolist())
-        assert float(outs[0][0].view(-1, 4)[:, 3].sum()) > 0
+        assert float(outs[1][0].view(-1, 4)[:, 3].sum()) > 0
+        if tiles and tiles[0][0] >= 40:
+            assert float(outs[0][0].abs().sum()) == 0.0 and float(outs[0][1].abs().sum()) == 0.0
     finally:
         for k in ks:
             k.Destroy()

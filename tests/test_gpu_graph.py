@@ -52,3 +52,4 @@ def test_graph_replays_equal_the_eager_call(pkg, gpu, entry):
         assert not torch.equal(other[0], ref[0])
     del g
     k.Destroy()
+

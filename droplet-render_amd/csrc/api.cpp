@@ -38,11 +38,29 @@ static int hip_fail(hipError_t e, const char *what) {
 static drk_alloc_gate_fn g_gate = nullptr;
 static void *g_gate_user = nullptr;
 
+// Debugging aid: with DRK_POISON_ALLOC=1 in the environment every device
+// buffer the library allocates starts filled with 0xff bytes (NaN floats,
+// all-ones integers), so that a kernel reading memory it was never given a
+// value for shows it in its results the same way every run. The fill is a
+// synchronous hipMemset (not for callers that capture graphs).
+static bool poison_alloc() {
+    static const bool on = [] {
+        const char *v = std::getenv("DRK_POISON_ALLOC");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+
 template <class T>
 static hipError_t dev_malloc(T **p, size_t bytes) {
     *p = nullptr;
     if (g_gate && g_gate(bytes, g_gate_user) != 0) return hipErrorOutOfMemory;
-    return hipMalloc(reinterpret_cast<void **>(p), bytes);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
+    if (e == hipSuccess && poison_alloc()) {
+        e = hipMemset(*p, 0xff, bytes);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    return e;
 }
 
 #define HIP_TRY(expr, what)                      \

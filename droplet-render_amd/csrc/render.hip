@@ -333,6 +333,12 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
         kAdj ? W.grad + (size_t)((xcc_id() + W.grad_nxcc * cu_slot()) % W.grad_copies) * W.grad_stride : nullptr;
 
     auto start_ray = [&](uint32_t ray_id) {
+#ifdef DRK_POISON_LDS
+        // Debug builds: the lane's stack and root interval read as all-ones
+        // or NaN until the new ray writes them.
+        for (uint32_t i = 0; i < kMaxDepth; ++i) stk[i * stride] = 0xffffffffu;
+        for (uint32_t i = 0; i < 6; ++i) root[i * stride] = __uint_as_float(0xffffffffu);
+#endif
         td = 0.0f;
         hit = false;
         T.lvl = -1;
@@ -361,6 +367,11 @@ __device__ __forceinline__ void render_body(const RenderParams &P, const WorkPar
     };
 
     auto start_path = [&]() {
+#ifdef DRK_POISON_LDS
+        // Debug builds: the lane's continuation slots read as NaN until an
+        // event of the new path writes them.
+        for (uint32_t i = 0; i < 9; ++i) cont[i * stride] = __uint_as_float(0xffffffffu);
+#endif
         bool gm;
         if (W.cam) {
             const float4 c = W.cam[pix_local];

@@ -102,6 +102,7 @@ _SIGS = {
     "drk_scene_load": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(HostScene))]),
     "drk_scene_create": (C.c_int, [C.POINTER(OctreeNode), u32, _F, u32, _F, u32, u32, C.POINTER(_P)]),
     "drk_scene_update_fog": (C.c_int, [_P, _F, _F]),
+    "drk_scene_update_fog_device": (C.c_int, [_P, _P, _P]),
     "drk_scene_destroy": (None, [_P]),
     "drk_kernel_create": (C.c_int, [_P, _F, _F, C.POINTER(SamplerDesc), _F, u32, f32, f32, u32, u32, u32, u32,
                                     u32, C.POINTER(_P)]),

@@ -78,9 +78,8 @@ class _RenderViews(torch.autograd.Function):
         assert all(k.scene is scene for k in kernels)
         assert density.is_cuda and density.dtype == torch.float32 and density.is_contiguous()
         assert density.numel() == scene.leafx[1] * scene.lvoxc3
-        torch.cuda.current_stream(density.device).synchronize()
-        scene.update_fog(density)
         stream = torch.cuda.current_stream(density.device)
+        scene.update_fog_device(density, stream=stream)   # ordered after density's producer
         outs, recs, jobs = [], [], []
         for k, seed in zip(kernels, seeds):
             n = k.tilex * k.tiley * 4
@@ -106,9 +105,8 @@ class _RenderViews(torch.autograd.Function):
         (density,) = ctx.saved_tensors
         kernels, samples = ctx.kernels, ctx.samples
         scene = kernels[0].scene
-        torch.cuda.current_stream(density.device).synchronize()
-        scene.update_fog(density)
         stream = torch.cuda.current_stream(density.device)
+        scene.update_fog_device(density, stream=stream)   # the adjoint replays these densities
         grad = torch.zeros_like(density)
         views = []
         for i, (k, seed) in enumerate(zip(kernels, ctx.seeds)):

@@ -390,6 +390,14 @@ int drk_scene_update_fog(drk_scene *s, const float *pvol_fog, const float *qval_
     return DRK_OK;
 }
 
+int drk_scene_update_fog_device(drk_scene *s, const float *d_pvol_fog, void *stream) {
+    if (!s || !d_pvol_fog) return set_error(DRK_EINVAL, "drk_scene_update_fog_device: NULL argument");
+    const size_t nb = (size_t)s->leafx[1] * s->lvoxc * s->lvoxc * s->lvoxc;
+    HIP_TRY(hipMemcpyAsync(s->d_fog, d_pvol_fog, nb * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)stream),
+            "drk_scene_update_fog_device");
+    return DRK_OK;
+}
+
 int drk_scene_set_reference_fog(drk_scene *s, const float *pvol_ref) {
     if (!s) return set_error(DRK_EINVAL, "drk_scene_set_reference_fog: NULL scene");
     if (!pvol_ref) {

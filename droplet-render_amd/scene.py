@@ -125,6 +125,14 @@ class Scene:
         if q is not None:
             self.ob["qval"][:, L.DRK_VOLUME_FOG] = q
 
+    def update_fog_device(self, pvol_fog, stream=None):
+        """Replaces the FOG bricks from a CUDA tensor with a copy on `stream`
+        (ordered with the stream's other work; no host synchronisation)."""
+        assert pvol_fog.is_cuda and pvol_fog.dtype.itemsize == 4 and pvol_fog.is_contiguous()
+        assert pvol_fog.numel() == self.leafx[1] * self.lvoxc3
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        L.check(L.lib().drk_scene_update_fog_device(self.device(), C.c_void_p(pvol_fog.data_ptr()), s))
+
     def set_walk(self, walk):
         """DRK_WALK_AUTO, DRK_WALK_OCTREE or DRK_WALK_GRID (next launches)."""
         L.check(L.lib().drk_scene_set_walk(self.device(), int(walk)))

@@ -112,6 +112,12 @@ int drk_scene_walk_info(const drk_scene *scene, uint32_t *grid_levels, float *oc
 /* Replaces the FOG bricks and per-leaf majorants in place (same leaf count);
  * used between optimisation steps. qval may be NULL to keep the majorants. */
 int  drk_scene_update_fog(drk_scene *scene, const float *pvol_fog, const float *qval_fog);
+/* Stream-ordered form for FOG bricks already in device memory (no reference
+ * counterpart): an asynchronous copy on `stream`, ordered after the work
+ * that produced d_pvol_fog and before later launches on that stream; the
+ * majorants are kept. Launches on other streams must be ordered by the
+ * caller. */
+int  drk_scene_update_fog_device(drk_scene *scene, const float *d_pvol_fog, void *stream);
 void drk_scene_destroy(drk_scene *scene);
 
 /* KernelSampler objects (src/KernelSampler.h:6-97). */

@@ -25,12 +25,11 @@ class _Render(torch.autograd.Function):
         scene = kernel.scene
         assert density.is_cuda and density.dtype == torch.float32 and density.is_contiguous()
         assert density.numel() == scene.leafx[1] * scene.lvoxc3
-        torch.cuda.current_stream(density.device).synchronize()
-        scene.update_fog(density)                        # device-to-device copy into the scene
+        stream = torch.cuda.current_stream(density.device)
+        scene.update_fog_device(density, stream=stream)   # device-to-device copy into the scene, in stream order
         n = tilex * tiley * 4
         out_l = torch.empty(n, dtype=torch.float32, device=density.device)
         out_e = torch.empty(n, dtype=torch.float32, device=density.device)
-        stream = torch.cuda.current_stream(density.device)
         ctx.rec = None
         if record:
             nb = kernel.record_bytes(x0, y0, tilex, tiley, samples)
@@ -49,12 +48,11 @@ class _Render(torch.autograd.Function):
         (density,) = ctx.saved_tensors
         kernel = ctx.kernel
         x0, y0, tilex, tiley, samples, seed, sample_base = ctx.args
-        torch.cuda.current_stream(density.device).synchronize()
-        kernel.scene.update_fog(density)                 # the adjoint replays these densities
+        stream = torch.cuda.current_stream(density.device)
+        kernel.scene.update_fog_device(density, stream=stream)   # the adjoint replays these densities
         g_l = torch.zeros(tiley, tilex, 4, device=density.device) if g_l is None else g_l.contiguous().float()
         g_e = torch.zeros(tiley, tilex, 4, device=density.device) if g_e is None else g_e.contiguous().float()
         grad = torch.zeros_like(density)
-        stream = torch.cuda.current_stream(density.device)
         if ctx.rec is not None:
             kernel.adjoint_record_device(x0, y0, tilex, tiley, samples, seed, sample_base, g_l, g_e, grad, ctx.rec,
                                          stream=stream)
